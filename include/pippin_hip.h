@@ -1,0 +1,235 @@
+/*
+ * pippin_hip.h — C ABI of the MI355X batched FP64 physics backend.
+ *
+ * This is the drop-in seam for Pippin's `Physics` trait
+ * (crates/pippin-env/src/lib.rs:110-155). A Rust FFI crate `impl Physics` over
+ * these entry points; AsyncEnv and the renderers stay unchanged because
+ * `scene()` is still built on the Rust side from the same `pippin::Model`.
+ *
+ * Conventions (mirroring the trait and PippinCpu, crates/pippin-env/src/pippin_cpu.rs):
+ *   - plain pointers and sizes only; every buffer is caller-owned host memory
+ *     unless a function says it takes device pointers;
+ *   - every function returns PH_OK (0) or a negative status; the text of the
+ *     last error of the calling thread is available from pippin_hip_last_error();
+ *   - a buffer-size mismatch returns PH_ERR_SIZE (the trait panics via
+ *     `check_len`, lib.rs:241-243; the FFI crate maps this status to a panic);
+ *   - set_param validates everything before touching any environment, so a
+ *     failing call leaves no partial update (pippin_cpu.rs:122-131);
+ *   - `step` is synchronous: it returns after the device finished the steps.
+ *   - Arrays of structures are row-major: (envs, width).
+ */
+#ifndef PIPPIN_HIP_H
+#define PIPPIN_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PH_OK = 0,
+  PH_ERR_ARG = -1,     /* invalid argument (ids, ranges, values) */
+  PH_ERR_SIZE = -2,    /* buffer length does not match envs x width */
+  PH_ERR_MODEL = -3,   /* model cannot be loaded or is unsupported */
+  PH_ERR_DEVICE = -4,  /* HIP runtime error */
+  PH_ERR_NOTPD = -5,   /* mass matrix not positive definite (forward.rs:397-399 panics) */
+};
+
+/* Joint types, in the order of crates/pippin/src/model.rs:7-12. */
+enum { PH_JNT_FREE = 0, PH_JNT_BALL = 1, PH_JNT_SLIDE = 2, PH_JNT_HINGE = 3 };
+/* Geom types, in the order of crates/pippin/src/model.rs:32-40 (the narrow
+ * phase orders pairs by this value, collision.rs:62). */
+enum { PH_GEOM_PLANE = 0, PH_GEOM_SPHERE = 1, PH_GEOM_CAPSULE = 2, PH_GEOM_BOX = 3,
+       PH_GEOM_CYLINDER = 4, PH_GEOM_MESH = 5 };
+enum { PH_INTEGRATOR_EULER = 0, PH_INTEGRATOR_IMPLICITFAST = 1 };
+enum { PH_SOLVER_NEWTON = 0, PH_SOLVER_PGS = 1 };
+/* `Field` of crates/pippin-env/src/lib.rs:22-27. */
+enum { PH_FIELD_QPOS = 0, PH_FIELD_QVEL = 1, PH_FIELD_CTRL = 2 };
+/* `Param` of crates/pippin/src/params.rs:10-39, same order. */
+enum {
+  PH_PARAM_GEOM_FRICTION = 0, PH_PARAM_GEOM_SIZE, PH_PARAM_GEOM_RGBA, PH_PARAM_GEOM_CONTYPE,
+  PH_PARAM_GEOM_CONAFFINITY, PH_PARAM_BODY_MASS, PH_PARAM_DOF_DAMPING, PH_PARAM_ACTUATOR_GAIN,
+  PH_PARAM_ACTUATOR_BIAS, PH_PARAM_BODY_INERTIA, PH_PARAM_BODY_IPOS, PH_PARAM_QPOS0
+};
+
+/*
+ * Raw model: the fields the MJCF/URDF compiler fills before `Model::compile`
+ * (crates/pippin/src/model.rs:103-249, crates/pippin/src/mjcf.rs:211-395).
+ * Derived data (dof tree, dof_blocks, weld ids, bounding boxes, collision
+ * pairs, invweight0) is recomputed by the library from these fields, exactly
+ * as `Model::compile` and `forward::set_const` do. Vectors are flattened:
+ * Vec3 -> 3 doubles, Quat -> 4 (w, x, y, z), Mat3 -> 9 row-major,
+ * bools -> int32 0/1, usize::MAX -> -1.
+ */
+typedef struct ph_model_desc {
+  int32_t nq, nv, nu, nbody, njnt, ngeom, ncam, neq, nexclude, nmoment;
+  double timestep;
+  double gravity[3];
+  int32_t integrator;
+  /* SolverOptions, model.rs:60-100 */
+  int32_t solver_kind, solver_iterations;
+  double tolerance, contact_margin, solref[2], solimp[5], impratio;
+  double contact_hertz, contact_damping_ratio, contact_slop;
+
+  /* bodies (body 0 is the world) */
+  const int32_t* body_parent;
+  const double* body_pos;      /* nbody x 3 */
+  const double* body_quat;     /* nbody x 4 */
+  const int32_t* body_jntadr;
+  const int32_t* body_jntnum;
+  const int32_t* body_dofadr;
+  const int32_t* body_dofnum;
+  const double* body_mass;
+  const double* body_ipos;     /* nbody x 3 */
+  const double* body_inertia;  /* nbody x 9 */
+  /* joints */
+  const int32_t* jnt_type;
+  const int32_t* jnt_body;
+  const int32_t* jnt_qposadr;
+  const int32_t* jnt_dofadr;
+  const double* jnt_pos;       /* njnt x 3 */
+  const double* jnt_axis;      /* njnt x 3 */
+  const int32_t* jnt_limited;
+  const double* jnt_range;     /* njnt x 2 */
+  const double* jnt_stiffness;
+  const double* jnt_springref;
+  /* dofs */
+  const int32_t* dof_body;
+  const int32_t* dof_jnt;
+  const double* dof_damping;
+  const double* dof_armature;
+  /* geoms */
+  const int32_t* geom_type;
+  const int32_t* geom_body;
+  const double* geom_pos;      /* ngeom x 3 */
+  const double* geom_quat;     /* ngeom x 4 */
+  const double* geom_size;     /* ngeom x 3 */
+  const double* geom_friction;
+  const uint32_t* geom_contype;
+  const uint32_t* geom_conaffinity;
+  const float* geom_rgba;      /* ngeom x 4 */
+  const int32_t* geom_group;
+  const int32_t* geom_dataid;  /* mesh index or -1 */
+  /* cameras */
+  const int32_t* cam_body;
+  const double* cam_pos;       /* ncam x 3 */
+  const double* cam_quat;      /* ncam x 4 */
+  const double* cam_fovy;
+  /* actuators: transmission = moment entries [adr, adr+num) of (joint, coef) */
+  const double* actuator_gear;
+  const double* actuator_gain;
+  const double* actuator_bias;        /* nu x 3 */
+  const int32_t* actuator_ctrllimited;
+  const double* actuator_ctrlrange;   /* nu x 2 */
+  const int32_t* actuator_forcelimited;
+  const double* actuator_forcerange;  /* nu x 2 */
+  const int32_t* actuator_moment_adr;
+  const int32_t* actuator_moment_num;
+  const int32_t* moment_jnt;          /* nmoment */
+  const double* moment_coef;          /* nmoment */
+  /* joint equality: (q1 - q1_0) = poly(q2 - q2_0) */
+  const int32_t* eq_joint1;
+  const int32_t* eq_joint2;           /* -1: fixed to a constant */
+  const double* eq_polycoef;          /* neq x 5 */
+  const double* eq_solref;            /* neq x 2 */
+  const double* eq_solimp;            /* neq x 5 */
+  /* <contact><exclude body1 body2> */
+  const int32_t* exclude_pairs;       /* nexclude x 2 */
+  const double* qpos0;                /* nq: kinematic reference */
+  const double* qpos_reset;           /* nq: state Data::reset returns to */
+} ph_model_desc;
+
+/* World pose, crates/pippin-env/src/lib.rs:102-107 (#[repr(C)]). */
+typedef struct ph_pose { float pos[3]; float mat[9]; } ph_pose;
+/* Per-env geom appearance, crates/pippin-env/src/lib.rs:159-165 (#[repr(C)]). */
+typedef struct ph_appearance { float rgba[4]; float scale[3]; float pad; } ph_appearance;
+/* `Dims`, crates/pippin-env/src/lib.rs:29-36. */
+typedef struct ph_dims { int64_t nq, nv, nu, ngeom, ncam; } ph_dims;
+/* One contact, crates/pippin/src/collision.rs:10-19 (geom ids, pair-local hit index). */
+typedef struct ph_contact {
+  double pos[3], normal[3], tangent[3], depth;
+  int32_t geom1, geom2, feature, pad;
+} ph_contact;
+
+typedef struct ph_sim ph_sim;   /* opaque handle: one batch of environments on one GPU */
+typedef struct ph_model ph_model; /* opaque handle: a raw model owned by the library */
+
+/* Options for pippin_hip_create. Zero-initialise and set what you need. */
+typedef struct ph_options {
+  int32_t device;           /* HIP device ordinal */
+  int32_t solver_override;  /* -1: keep the model's; else PH_SOLVER_* */
+  int32_t max_contacts;     /* per-env contact capacity; 0 = derived from the model */
+  int32_t reserved[5];
+} ph_options;
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* pippin_hip_last_error(void);
+
+/* ---- model loading (host; replaces pippin::mjcf::load_file, mjcf.rs:161-170) ---- */
+/* Parse an MJCF document (subset of mjcf.rs). `dir` resolves relative asset paths. */
+int pippin_hip_model_load_xml(const char* xml, const char* dir, ph_model** out);
+int pippin_hip_model_load_file(const char* path, ph_model** out);
+/* Borrow the flat descriptor of a loaded model (valid until pippin_hip_model_free). */
+const ph_model_desc* pippin_hip_model_desc(const ph_model* m);
+void pippin_hip_model_free(ph_model* m);
+
+/* ---- lifecycle: PippinCpu::new (pippin_cpu.rs:267-273) ---- */
+/* Copies `desc` (the caller may free it afterwards), compiles derived data,
+ * uploads it and computes invweight0 on the device. */
+int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, const ph_options* opt, ph_sim** out);
+void pippin_hip_destroy(ph_sim* s);
+int pippin_hip_dims(const ph_sim* s, ph_dims* out);
+int64_t pippin_hip_num_envs(const ph_sim* s);
+
+/* ---- Physics::step / reset / get / set / poses (lib.rs:116-128) ---- */
+/* Advance envs [begin, end) by nstep steps, ctrl held; then refresh poses. */
+int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep);
+int pippin_hip_reset(ph_sim* s, const int64_t* envs, int64_t n);
+/* `out` / `src` hold (end - begin) x width doubles, width = dims.{nq,nv,nu}. */
+int pippin_hip_get(const ph_sim* s, int32_t field, int64_t begin, int64_t end, double* out, int64_t len);
+int pippin_hip_set(ph_sim* s, int32_t field, int64_t begin, int64_t end, const double* src, int64_t len);
+/* Geom poses (envs x ngeom) and camera poses (envs x ncam). */
+int pippin_hip_poses(const ph_sim* s, int64_t begin, int64_t end, ph_pose* geoms, int64_t ngeoms_len,
+                     ph_pose* cams, int64_t ncams_len);
+
+/* ---- per-env parameters: Physics::{set,get}_param (lib.rs:130-141), params.rs:111-177 ---- */
+/* `values` holds (end - begin) x width doubles (width = Param::width_in). */
+int pippin_hip_set_param(ph_sim* s, int32_t param, int64_t id, int64_t begin, int64_t end,
+                         const double* values, int64_t len);
+int pippin_hip_get_param(const ph_sim* s, int32_t param, int64_t id, int64_t begin, int64_t end,
+                         double* out, int64_t len);
+int pippin_hip_appearance(const ph_sim* s, int64_t begin, int64_t end, ph_appearance* out, int64_t len);
+
+/* ---- diagnostics for parity and measurement ---- */
+/* Contacts of the last step per env (Sim.ncon, pippin-py/src/lib.rs:171-173). */
+int pippin_hip_ncon(const ph_sim* s, int64_t begin, int64_t end, int32_t* out, int64_t len);
+/* Contacts of env `env`; writes at most `cap`, returns the count in *n. */
+int pippin_hip_contacts(const ph_sim* s, int64_t env, ph_contact* out, int32_t cap, int32_t* n);
+/* Newton iterations of the last step per env (Data::solver_iterations, data.rs:53-55). */
+int pippin_hip_solver_iterations(const ph_sim* s, int64_t begin, int64_t end, int32_t* out, int64_t len);
+/* Run forward dynamics (kinematics, CRBA, RNE; no stepping) and copy M(q)
+ * (envs x nv x nv) and c(q, qdot) = qfrc_bias (envs x nv). */
+int pippin_hip_mass_bias(ph_sim* s, int64_t begin, int64_t end, double* qm, int64_t qm_len, double* bias,
+                         int64_t bias_len);
+/* dof_invweight (envs x nv) and body_invweight (envs x nbody x 2) as set_const computes them. */
+int pippin_hip_invweight(const ph_sim* s, int64_t begin, int64_t end, double* dof, int64_t dof_len,
+                         double* body, int64_t body_len);
+/* Number of candidate collision pairs and their (g1, g2) list (compile-time filtering). */
+int pippin_hip_collision_pairs(const ph_sim* s, int64_t env, int32_t* pairs, int32_t cap, int32_t* n);
+
+/* ---- device-resident rollout (Batch::rollout, crates/pippin/src/batch.rs:46-59) ---- */
+/* All pointers are DEVICE pointers: ctrl (envs x nstep x nu), qpos_out (envs x nstep x nq)
+ * for envs [begin, end). Runs on the library's stream and synchronises. */
+int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const double* ctrl,
+                              double* qpos_out);
+/* Device pointers of the resident state (envs x width), for zero-copy callers. */
+int pippin_hip_state_device(ph_sim* s, int32_t field, double** ptr);
+/* Stream the library launches on (a hipStream_t). */
+void* pippin_hip_stream(ph_sim* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIPPIN_HIP_H */

@@ -1,0 +1,307 @@
+// ORACLE — test infrastructure only. Nothing in the product (pippin_amd/, include/)
+// may include, link or call this code; only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py use it, as the checker.
+//
+// Plain-old-data 3-D and spatial algebra restating crates/pippin/src/math.rs and
+// crates/pippin/src/simd.rs (x86 branch) operation for operation. Build with
+// -ffp-contract=off and without fast-math: the Rust reference never contracts
+// a*b+c into an FMA, and the order of every sum below is the reference's.
+#pragma once
+#include <cmath>
+#include <cstddef>
+
+namespace orc {
+
+typedef double Real;
+
+// Rust's `impl Sum for f64` folds from -0.0; `rsum_start()` keeps that sign rule.
+static inline Real rsum_start() { return -0.0; }
+
+// Rust f64::clamp: `if x < lo {x = lo}; if x > hi {x = hi}` (NaN passes through).
+static inline Real rclamp(Real x, Real lo, Real hi) {
+  if (x < lo) x = lo;
+  if (x > hi) x = hi;
+  return x;
+}
+// Rust f64::max / f64::min (IEEE maxNum/minNum: a NaN operand yields the other).
+static inline Real rmax(Real a, Real b) { return std::fmax(a, b); }
+static inline Real rmin(Real a, Real b) { return std::fmin(a, b); }
+// Rust f64::signum: +1 for +0.0 and positives, -1 for -0.0 and negatives.
+static inline Real rsignum(Real x) { return std::isnan(x) ? x : std::copysign(1.0, x); }
+
+struct Vec3 {
+  Real v[3];
+  Real& operator[](int i) { return v[i]; }
+  Real operator[](int i) const { return v[i]; }
+};
+static inline Vec3 V3(Real x, Real y, Real z) { Vec3 r = {{x, y, z}}; return r; }
+static inline Vec3 v3zero() { return V3(0.0, 0.0, 0.0); }
+static inline Vec3 operator+(Vec3 a, Vec3 b) { return V3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+static inline Vec3 operator-(Vec3 a, Vec3 b) { return V3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+static inline Vec3 operator-(Vec3 a) { return V3(-a[0], -a[1], -a[2]); }
+static inline Vec3 operator*(Vec3 a, Real s) { return V3(a[0] * s, a[1] * s, a[2] * s); }
+static inline Vec3& operator+=(Vec3& a, Vec3 b) { a = a + b; return a; }
+static inline Vec3& operator-=(Vec3& a, Vec3 b) { a = a - b; return a; }
+// math.rs:22-24
+static inline Real dot(Vec3 a, Vec3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+// math.rs:26-30
+static inline Vec3 cross(Vec3 a, Vec3 o) {
+  return V3(a[1] * o[2] - a[2] * o[1], a[2] * o[0] - a[0] * o[2], a[0] * o[1] - a[1] * o[0]);
+}
+static inline Real norm2(Vec3 a) { return dot(a, a); }
+static inline Real norm(Vec3 a) { return std::sqrt(norm2(a)); }
+// math.rs:40-47
+static inline Vec3 normalized(Vec3 a) {
+  Real n = norm(a);
+  if (n > 1e-15) return a * (1.0 / n);
+  return v3zero();
+}
+static inline Vec3 vabs(Vec3 a) { return V3(std::fabs(a[0]), std::fabs(a[1]), std::fabs(a[2])); }
+static inline Vec3 mul_elem(Vec3 a, Vec3 o) { return V3(a[0] * o[0], a[1] * o[1], a[2] * o[2]); }
+static inline Vec3 vmin(Vec3 a, Vec3 o) { return V3(rmin(a[0], o[0]), rmin(a[1], o[1]), rmin(a[2], o[2])); }
+static inline Vec3 vmax(Vec3 a, Vec3 o) { return V3(rmax(a[0], o[0]), rmax(a[1], o[1]), rmax(a[2], o[2])); }
+// math.rs:65-70
+static inline void tangents(Vec3 n, Vec3* t1, Vec3* t2) {
+  Vec3 a = std::fabs(n[0]) < 0.57 ? V3(1, 0, 0) : V3(0, 1, 0);
+  *t1 = normalized(cross(n, a));
+  *t2 = cross(n, *t1);
+}
+
+// Row-major 3x3 (math.rs:127-199).
+struct Mat3 {
+  Real m[9];
+  Real& operator[](int i) { return m[i]; }
+  Real operator[](int i) const { return m[i]; }
+};
+static inline Mat3 m3identity() { Mat3 r = {{1, 0, 0, 0, 1, 0, 0, 0, 1}}; return r; }
+static inline Mat3 m3zero() { Mat3 r = {{0, 0, 0, 0, 0, 0, 0, 0, 0}}; return r; }
+static inline Mat3 m3diag(Vec3 d) { Mat3 r = {{d[0], 0, 0, 0, d[1], 0, 0, 0, d[2]}}; return r; }
+static inline Mat3 from_cols(Vec3 a, Vec3 b, Vec3 c) {
+  Mat3 r = {{a[0], b[0], c[0], a[1], b[1], c[1], a[2], b[2], c[2]}};
+  return r;
+}
+static inline Vec3 col(const Mat3& m, int j) { return V3(m[j], m[3 + j], m[6 + j]); }
+static inline Vec3 mul_vec(const Mat3& m, Vec3 v) {
+  return V3(m[0] * v[0] + m[1] * v[1] + m[2] * v[2], m[3] * v[0] + m[4] * v[1] + m[5] * v[2],
+            m[6] * v[0] + m[7] * v[1] + m[8] * v[2]);
+}
+static inline Vec3 tmul_vec(const Mat3& m, Vec3 v) {
+  return V3(m[0] * v[0] + m[3] * v[1] + m[6] * v[2], m[1] * v[0] + m[4] * v[1] + m[7] * v[2],
+            m[2] * v[0] + m[5] * v[1] + m[8] * v[2]);
+}
+// math.rs:169-177: each entry is an iterator sum (starts at -0.0, then k = 0, 1, 2).
+static inline Mat3 mul_mat(const Mat3& a, const Mat3& o) {
+  Mat3 r;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      Real s = rsum_start();
+      for (int k = 0; k < 3; k++) s += a[3 * i + k] * o[3 * k + j];
+      r[3 * i + j] = s;
+    }
+  return r;
+}
+static inline Mat3 transpose(const Mat3& m) {
+  Mat3 r = {{m[0], m[3], m[6], m[1], m[4], m[7], m[2], m[5], m[8]}};
+  return r;
+}
+static inline Mat3 skew(Vec3 v) {
+  Mat3 r = {{0.0, -v[2], v[1], v[2], 0.0, -v[0], -v[1], v[0], 0.0}};
+  return r;
+}
+static inline Mat3 mscale(const Mat3& m, Real s) {
+  Mat3 r;
+  for (int i = 0; i < 9; i++) r[i] = m[i] * s;
+  return r;
+}
+static inline Mat3 madd(const Mat3& a, const Mat3& o) {
+  Mat3 r;
+  for (int i = 0; i < 9; i++) r[i] = a[i] + o[i];
+  return r;
+}
+// math.rs:196-198: R * self * R^T
+static inline Mat3 mrotate(const Mat3& self, const Mat3& r) { return mul_mat(mul_mat(r, self), transpose(r)); }
+
+// Unit quaternion (w, x, y, z), math.rs:201-301.
+struct Quat {
+  Real q[4];
+  Real& operator[](int i) { return q[i]; }
+  Real operator[](int i) const { return q[i]; }
+};
+static inline Quat Q4(Real w, Real x, Real y, Real z) { Quat r = {{w, x, y, z}}; return r; }
+static inline Quat qidentity() { return Q4(1, 0, 0, 0); }
+static inline Quat from_axis_angle(Vec3 axis, Real angle) {
+  Vec3 a = normalized(axis);
+  Real s = std::sin(0.5 * angle), c = std::cos(0.5 * angle);
+  return Q4(c, a[0] * s, a[1] * s, a[2] * s);
+}
+static inline Quat qnormalized(Quat q) {
+  Real s = rsum_start();
+  for (int i = 0; i < 4; i++) s += q[i] * q[i];
+  Real n = std::sqrt(s);
+  if (n < 1e-15) return qidentity();
+  return Q4(q[0] / n, q[1] / n, q[2] / n, q[3] / n);
+}
+static inline Quat qmul(Quat p, Quat o) {
+  Real a = p[0], b = p[1], c = p[2], d = p[3];
+  Real e = o[0], f = o[1], g = o[2], h = o[3];
+  return Q4(a * e - b * f - c * g - d * h, a * f + b * e + c * h - d * g, a * g - b * h + c * e + d * f,
+            a * h + b * g - c * f + d * e);
+}
+static inline Mat3 to_mat(Quat q) {
+  Real w = q[0], x = q[1], y = q[2], z = q[3];
+  Real ww = w * w, xx = x * x, yy = y * y, zz = z * z;
+  Real xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y, wz = w * z;
+  Mat3 r = {{ww + xx - yy - zz, 2.0 * (xy - wz), 2.0 * (xz + wy), 2.0 * (xy + wz), ww - xx + yy - zz,
+             2.0 * (yz - wx), 2.0 * (xz - wy), 2.0 * (yz + wx), ww - xx - yy + zz}};
+  return r;
+}
+static inline Vec3 qrotate(Quat q, Vec3 v) { return mul_vec(to_mat(q), v); }
+// Shepperd's method, math.rs:260-282
+static inline Quat from_mat(const Mat3& m) {
+  Real tr = m[0] + m[4] + m[8];
+  Quat q;
+  if (tr > 0.0) {
+    Real s = std::sqrt(tr + 1.0) * 2.0;
+    q = Q4(0.25 * s, (m[7] - m[5]) / s, (m[2] - m[6]) / s, (m[3] - m[1]) / s);
+  } else if (m[0] > m[4] && m[0] > m[8]) {
+    Real s = std::sqrt(1.0 + m[0] - m[4] - m[8]) * 2.0;
+    q = Q4((m[7] - m[5]) / s, 0.25 * s, (m[1] + m[3]) / s, (m[2] + m[6]) / s);
+  } else if (m[4] > m[8]) {
+    Real s = std::sqrt(1.0 + m[4] - m[0] - m[8]) * 2.0;
+    q = Q4((m[2] - m[6]) / s, (m[1] + m[3]) / s, 0.25 * s, (m[5] + m[7]) / s);
+  } else {
+    Real s = std::sqrt(1.0 + m[8] - m[0] - m[4]) * 2.0;
+    q = Q4((m[3] - m[1]) / s, (m[2] + m[6]) / s, (m[5] + m[7]) / s, 0.25 * s);
+  }
+  q = qnormalized(q);
+  if (q[0] < 0.0) q = Q4(-q[0], -q[1], -q[2], -q[3]);
+  return q;
+}
+// math.rs:284-292
+static inline Quat from_z_to(Vec3 dir) {
+  Vec3 d = normalized(dir);
+  Real c = dot(V3(0, 0, 1), d);
+  if (c < -1.0 + 1e-12) return Q4(0.0, 1.0, 0.0, 0.0);
+  Vec3 axis = cross(V3(0, 0, 1), d);
+  return qnormalized(Q4(1.0 + c, axis[0], axis[1], axis[2]));
+}
+// math.rs:294-300
+static inline Quat integrate_local(Quat self, Vec3 omega, Real h) {
+  Real n = norm(omega);
+  if (n * h < 1e-15) return self;
+  return qnormalized(qmul(self, from_axis_angle(omega * (1.0 / n), n * h)));
+}
+
+// Spatial motion/force [ang; lin] about the world origin, math.rs:303-369.
+struct Spatial {
+  Vec3 ang, lin;
+};
+static inline Spatial SP(Vec3 a, Vec3 l) { Spatial s; s.ang = a; s.lin = l; return s; }
+static inline Spatial spzero() { return SP(v3zero(), v3zero()); }
+static inline Real sdot(const Spatial& a, const Spatial& o) { return dot(a.ang, o.ang) + dot(a.lin, o.lin); }
+static inline Spatial sscale(const Spatial& a, Real s) { return SP(a.ang * s, a.lin * s); }
+static inline Spatial cross_motion(const Spatial& s, const Spatial& m) {
+  return SP(cross(s.ang, m.ang), cross(s.ang, m.lin) + cross(s.lin, m.ang));
+}
+static inline Spatial cross_force(const Spatial& s, const Spatial& f) {
+  return SP(cross(s.ang, f.ang) + cross(s.lin, f.lin), cross(s.ang, f.lin));
+}
+static inline Vec3 point_velocity(const Spatial& s, Vec3 p) { return s.lin + cross(s.ang, p); }
+static inline Spatial operator+(const Spatial& a, const Spatial& o) { return SP(a.ang + o.ang, a.lin + o.lin); }
+static inline Spatial& operator+=(Spatial& a, const Spatial& o) {
+  a.ang += o.ang;
+  a.lin += o.lin;
+  return a;
+}
+
+// math.rs:371-410
+struct SpatialInertia {
+  Real mass;
+  Vec3 mc;
+  Mat3 inertia_o;
+};
+static inline SpatialInertia sizero() {
+  SpatialInertia s;
+  s.mass = 0.0;
+  s.mc = v3zero();
+  s.inertia_o = m3zero();
+  return s;
+}
+static inline SpatialInertia from_com(Real mass, Vec3 com, const Mat3& inertia_com) {
+  Mat3 cx = skew(com);
+  SpatialInertia s;
+  s.inertia_o = madd(inertia_com, mscale(mul_mat(cx, transpose(cx)), mass));
+  s.mass = mass;
+  s.mc = com * mass;
+  return s;
+}
+static inline SpatialInertia siadd(const SpatialInertia& a, const SpatialInertia& o) {
+  SpatialInertia s;
+  s.mass = a.mass + o.mass;
+  s.mc = a.mc + o.mc;
+  s.inertia_o = madd(a.inertia_o, o.inertia_o);
+  return s;
+}
+static inline Spatial mul_motion(const SpatialInertia& I, const Spatial& v) {
+  return SP(mul_vec(I.inertia_o, v.ang) + cross(I.mc, v.lin), v.lin * I.mass - cross(I.mc, v.ang));
+}
+
+// simd.rs:14-64, x86 branch: four strided accumulators when n >= 8.
+static inline Real sdot_n(const Real* a, const Real* b, size_t n) {
+  if (n < 8) {
+    Real s = 0.0;
+    for (size_t k = 0; k < n; k++) s += a[k] * b[k];
+    return s;
+  }
+  Real s[4] = {0.0, 0.0, 0.0, 0.0};
+  size_t k = 0;
+  while (k + 4 <= n) {
+    for (int j = 0; j < 4; j++) s[j] += a[k + j] * b[k + j];
+    k += 4;
+  }
+  Real t = (s[0] + s[1]) + (s[2] + s[3]);
+  while (k < n) {
+    t += a[k] * b[k];
+    k++;
+  }
+  return t;
+}
+// simd.rs:68-72
+static inline void axpy(Real alpha, const Real* x, Real* y, size_t n) {
+  for (size_t k = 0; k < n; k++) y[k] += alpha * x[k];
+}
+
+// math.rs:425-447: in-place lower Cholesky of the diagonal block [s0, e) of a
+// row-major matrix with row stride `stride`.
+static inline bool cholesky_block(Real* a, size_t stride, size_t s0, size_t e) {
+  for (size_t j = s0; j < e; j++) {
+    Real* rj = a + j * stride + s0;
+    size_t len = j - s0;
+    Real d = a[j * stride + j] - sdot_n(rj, rj, len);
+    if (d <= 0.0) return false;
+    d = std::sqrt(d);
+    a[j * stride + j] = d;
+    Real inv = 1.0 / d;
+    for (size_t i = j + 1; i < e; i++) {
+      Real* ri = a + i * stride + s0;
+      size_t k = j - s0;
+      ri[k] = (ri[k] - sdot_n(ri, rj, k)) * inv;
+    }
+  }
+  return true;
+}
+// math.rs:451-463
+static inline void cholesky_solve_block(const Real* l, size_t stride, size_t s0, size_t e, Real* x) {
+  for (size_t i = s0; i < e; i++)
+    x[i] = (x[i] - sdot_n(l + i * stride + s0, x + s0, i - s0)) / l[i * stride + i];
+  for (size_t ii = e; ii > s0; ii--) {
+    size_t i = ii - 1;
+    x[i] /= l[i * stride + i];
+    Real xi = x[i];
+    axpy(-xi, l + i * stride + s0, x + s0, i - s0);
+  }
+}
+static inline bool cholesky(Real* a, size_t n) { return cholesky_block(a, n, 0, n); }
+static inline void cholesky_solve(const Real* l, size_t n, Real* x) { cholesky_solve_block(l, n, 0, n, x); }
+
+}  // namespace orc
