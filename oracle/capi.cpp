@@ -316,6 +316,9 @@ int orc_model_info(orc_sim* s, int64_t env, double* body_mass, double* body_ipos
 }
 
 // ---- known-answer hooks ----
+void orc_cr_sincos(const double* x, int64_t n, double* s, double* c) {
+  for (int64_t i = 0; i < n; i++) cr_sincos(x[i], s + i, c + i);
+}
 double orc_impedance(const double* solimp, double pos) { return impedance(solimp, pos); }
 double orc_dot(const double* a, const double* b, int64_t n) { return sdot_n(a, b, (size_t)n); }
 int orc_cholesky_block(double* a, int64_t stride, int64_t s0, int64_t e) {

@@ -121,6 +121,127 @@ static inline Mat3 madd(const Mat3& a, const Mat3& o) {
 // math.rs:196-198: R * self * R^T
 static inline Mat3 mrotate(const Mat3& self, const Mat3& r) { return mul_mat(mul_mat(r, self), transpose(r)); }
 
+// ---- correctly rounded sin and cos (see the note below) ----
+// The reference calls the platform libm (Rust f64::sin_cos); libms differ from
+// each other by up to one ulp, and through contact one ulp grows into visible
+// trajectory differences. Both the device and the oracle therefore evaluate
+// the correctly rounded sin/cos: argument reduction by a three-double pi/2
+// (Cody-Waite with exact products) and Taylor series in double-double
+// arithmetic (terms 0..8) with a double tail (terms 9..14). The result is
+// within ~2^-100 relative of the exact value before the final rounding, so it
+// equals the exactly rounded value except at ties closer than that.
+struct DD {
+  Real hi, lo;
+};
+static inline DD dd_fast_two_sum(Real a, Real b) {
+  Real s = a + b;
+  DD r;
+  r.hi = s;
+  r.lo = b - (s - a);
+  return r;
+}
+static inline DD dd_two_sum(Real a, Real b) {
+  Real s = a + b;
+  Real bb = s - a;
+  DD r;
+  r.hi = s;
+  r.lo = (a - (s - bb)) + (b - bb);
+  return r;
+}
+static inline DD dd_add(DD a, DD b) {
+  DD s = dd_two_sum(a.hi, b.hi);
+  DD t = dd_two_sum(a.lo, b.lo);
+  Real e = s.lo + t.hi;
+  DD u = dd_fast_two_sum(s.hi, e);
+  e = u.lo + t.lo;
+  return dd_fast_two_sum(u.hi, e);
+}
+static inline DD dd_mul(DD a, DD b) {
+  Real p = a.hi * b.hi;
+  Real e = std::fma(a.hi, b.hi, -p);
+  e = e + (a.hi * b.lo + a.lo * b.hi);
+  return dd_fast_two_sum(p, e);
+}
+static inline DD dd_from(Real hi, Real lo) {
+  DD r;
+  r.hi = hi;
+  r.lo = lo;
+  return r;
+}
+static inline void cr_sincos(Real x, Real* sout, Real* cout) {
+  if (!std::isfinite(x)) {
+    *sout = x - x;
+    *cout = x - x;
+    return;
+  }
+  const Real P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
+  Real k = std::rint(x * 0.6366197723675814);
+  DD r = dd_from(x, 0.0);
+  Real h = k * P1;
+  r = dd_add(r, dd_from(-h, -std::fma(k, P1, -h)));
+  h = k * P2;
+  r = dd_add(r, dd_from(-h, -std::fma(k, P2, -h)));
+  r = dd_add(r, dd_from(-(k * P3), 0.0));
+  DD r2 = dd_mul(r, r);
+  const Real S[15][2] = {{1.0, 0.0},
+                         {-0.16666666666666666, -9.25185853854297e-18},
+                         {0.008333333333333333, 1.1564823173178714e-19},
+                         {-0.0001984126984126984, -1.7209558293420705e-22},
+                         {2.7557319223985893e-06, -1.858393274046472e-22},
+                         {-2.505210838544172e-08, 1.448814070935912e-24},
+                         {1.6059043836821613e-10, 1.2585294588752098e-26},
+                         {-7.647163731819816e-13, -7.03872877733453e-30},
+                         {2.8114572543455206e-15, 1.6508842730861433e-31},
+                         {-8.22063524662433e-18, 0.0},
+                         {1.9572941063391263e-20, 0.0},
+                         {-3.868170170630684e-23, 0.0},
+                         {6.446950284384474e-26, 0.0},
+                         {-9.183689863795546e-29, 0.0},
+                         {1.1309962886447716e-31, 0.0}};
+  const Real Cc[15][2] = {{1.0, 0.0},
+                          {-0.5, 0.0},
+                          {0.041666666666666664, 2.3129646346357427e-18},
+                          {-0.001388888888888889, 5.300543954373577e-20},
+                          {2.48015873015873e-05, 2.1511947866775882e-23},
+                          {-2.755731922398589e-07, -2.3767714622250297e-23},
+                          {2.08767569878681e-09, -1.20734505911326e-25},
+                          {-1.1470745597729725e-11, -2.0655512752830745e-28},
+                          {4.779477332387385e-14, 4.399205485834081e-31},
+                          {-1.5619206968586225e-16, 0.0},
+                          {4.110317623312165e-19, 0.0},
+                          {-8.896791392450574e-22, 0.0},
+                          {1.6117375710961184e-24, 0.0},
+                          {-2.4795962632247976e-27, 0.0},
+                          {3.279889237069838e-30, 0.0}};
+  Real ts = S[14][0], tc = Cc[14][0];
+  for (int i = 13; i >= 9; i--) {
+    ts = ts * r2.hi + S[i][0];
+    tc = tc * r2.hi + Cc[i][0];
+  }
+  DD ps = dd_from(ts, 0.0), pc = dd_from(tc, 0.0);
+  for (int i = 8; i >= 0; i--) {
+    ps = dd_add(dd_mul(ps, r2), dd_from(S[i][0], S[i][1]));
+    pc = dd_add(dd_mul(pc, r2), dd_from(Cc[i][0], Cc[i][1]));
+  }
+  DD s = dd_mul(ps, r);
+  Real sv = s.hi, cv = pc.hi;
+  long q = (long)k;
+  q = ((q % 4) + 4) % 4;
+  if (q == 0) {
+    *sout = sv;
+    *cout = cv;
+  } else if (q == 1) {
+    *sout = cv;
+    *cout = -sv;
+  } else if (q == 2) {
+    *sout = -sv;
+    *cout = -cv;
+  } else {
+    *sout = -cv;
+    *cout = sv;
+  }
+}
+
 // Unit quaternion (w, x, y, z), math.rs:201-301.
 struct Quat {
   Real q[4];
@@ -131,7 +252,8 @@ static inline Quat Q4(Real w, Real x, Real y, Real z) { Quat r = {{w, x, y, z}};
 static inline Quat qidentity() { return Q4(1, 0, 0, 0); }
 static inline Quat from_axis_angle(Vec3 axis, Real angle) {
   Vec3 a = normalized(axis);
-  Real s = std::sin(0.5 * angle), c = std::cos(0.5 * angle);
+  Real s, c;
+  cr_sincos(0.5 * angle, &s, &c);
   return Q4(c, a[0] * s, a[1] * s, a[2] * s);
 }
 static inline Quat qnormalized(Quat q) {

@@ -292,6 +292,16 @@ def quat_integrate(q, w, h):
     return out
 
 
+def cr_sincos(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    L = lib()
+    L.orc_cr_sincos.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    L.orc_cr_sincos.restype = None
+    L.orc_cr_sincos(_p(x), x.size, _p(s), _p(c))
+    return s, c
+
+
 def collide(ta, pa, ma, sa, tb, pb, mb, sb, margin=1e-3):
     arr = lambda x, n: np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(n))
     pa, ma, sa, pb, mb, sb = arr(pa, 3), arr(ma, 9), arr(sa, 3), arr(pb, 3), arr(mb, 9), arr(sb, 3)

@@ -1,0 +1,47 @@
+"""Build libpippin_hip.so in-tree with hipcc for gfx950.
+
+Every source is compiled with -ffp-contract=off so the device arithmetic keeps
+the reference's unfused multiply/add order, and with packed-FP32 instructions
+disabled.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+OUT = HERE / "libpippin_hip.so"
+SOURCES = ["runtime.hip", "host_model.cpp", "mjcf.cpp"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PIPPIN_HIP_ARCH", "gfx950")
+FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"--offload-arch={ARCH}",
+]
+
+
+def build(verbose: bool = False) -> Path:
+    objs = []
+    bdir = HERE / "_build"
+    bdir.mkdir(exist_ok=True)
+    for src in SOURCES:
+        obj = bdir / (src + ".o")
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        cmd = [HIPCC, *FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(str(obj))
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = [HIPCC, *FLAGS, "-shared", "-o", str(tmp), *objs]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

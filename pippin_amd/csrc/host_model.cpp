@@ -1,0 +1,442 @@
+// Host model build: copy of the raw descriptor and the derived data of
+// crates/pippin/src/model.rs::compile (279-316), in the device's layout.
+#include "host_model.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <set>
+
+namespace phh {
+
+template <class T>
+static void cp(std::vector<T>& v, const T* p, int64_t n) {
+  v.assign(p, p + (n > 0 ? n : 0));
+}
+
+void retarget(HostModel& m) {
+  ph_model_desc& d = m.d;
+  d.body_parent = m.body_parent.data();
+  d.body_pos = m.body_pos.data();
+  d.body_quat = m.body_quat.data();
+  d.body_jntadr = m.body_jntadr.data();
+  d.body_jntnum = m.body_jntnum.data();
+  d.body_dofadr = m.body_dofadr.data();
+  d.body_dofnum = m.body_dofnum.data();
+  d.body_mass = m.body_mass.data();
+  d.body_ipos = m.body_ipos.data();
+  d.body_inertia = m.body_inertia.data();
+  d.jnt_type = m.jnt_type.data();
+  d.jnt_body = m.jnt_body.data();
+  d.jnt_qposadr = m.jnt_qposadr.data();
+  d.jnt_dofadr = m.jnt_dofadr.data();
+  d.jnt_pos = m.jnt_pos.data();
+  d.jnt_axis = m.jnt_axis.data();
+  d.jnt_limited = m.jnt_limited.data();
+  d.jnt_range = m.jnt_range.data();
+  d.jnt_stiffness = m.jnt_stiffness.data();
+  d.jnt_springref = m.jnt_springref.data();
+  d.dof_body = m.dof_body.data();
+  d.dof_jnt = m.dof_jnt.data();
+  d.dof_damping = m.dof_damping.data();
+  d.dof_armature = m.dof_armature.data();
+  d.geom_type = m.geom_type.data();
+  d.geom_body = m.geom_body.data();
+  d.geom_pos = m.geom_pos.data();
+  d.geom_quat = m.geom_quat.data();
+  d.geom_size = m.geom_size.data();
+  d.geom_friction = m.geom_friction.data();
+  d.geom_contype = m.geom_contype.data();
+  d.geom_conaffinity = m.geom_conaffinity.data();
+  d.geom_rgba = m.geom_rgba.data();
+  d.geom_group = m.geom_group.data();
+  d.geom_dataid = m.geom_dataid.data();
+  d.cam_body = m.cam_body.data();
+  d.cam_pos = m.cam_pos.data();
+  d.cam_quat = m.cam_quat.data();
+  d.cam_fovy = m.cam_fovy.data();
+  d.actuator_gear = m.actuator_gear.data();
+  d.actuator_gain = m.actuator_gain.data();
+  d.actuator_bias = m.actuator_bias.data();
+  d.actuator_ctrllimited = m.actuator_ctrllimited.data();
+  d.actuator_ctrlrange = m.actuator_ctrlrange.data();
+  d.actuator_forcelimited = m.actuator_forcelimited.data();
+  d.actuator_forcerange = m.actuator_forcerange.data();
+  d.actuator_moment_adr = m.actuator_moment_adr.data();
+  d.actuator_moment_num = m.actuator_moment_num.data();
+  d.moment_jnt = m.moment_jnt.data();
+  d.moment_coef = m.moment_coef.data();
+  d.eq_joint1 = m.eq_joint1.data();
+  d.eq_joint2 = m.eq_joint2.data();
+  d.eq_polycoef = m.eq_polycoef.data();
+  d.eq_solref = m.eq_solref.data();
+  d.eq_solimp = m.eq_solimp.data();
+  d.exclude_pairs = m.exclude_pairs.data();
+  d.qpos0 = m.qpos0.data();
+  d.qpos_reset = m.qpos_reset.data();
+}
+
+void local_aabb(int t, const double* s, double* o) {
+  const double inf = INFINITY;
+  o[0] = o[1] = o[2] = 0.0;
+  switch (t) {
+    case PH_GEOM_PLANE: o[3] = o[4] = o[5] = inf; break;
+    case PH_GEOM_SPHERE: o[3] = o[4] = o[5] = s[0]; break;
+    case PH_GEOM_CAPSULE: o[3] = s[0]; o[4] = s[0]; o[5] = s[0] + s[1]; break;
+    case PH_GEOM_BOX: o[3] = s[0]; o[4] = s[1]; o[5] = s[2]; break;
+    case PH_GEOM_CYLINDER: o[3] = s[0]; o[4] = s[0]; o[5] = s[1]; break;
+    default: o[3] = o[4] = o[5] = 0.0;
+  }
+}
+
+bool has_implicit_terms(const HostModel& m, const double* damping, const double* bias) {
+  for (int k = 0; k < m.d.nv; k++)
+    if (damping[k] != 0.0) return true;
+  if (m.d.integrator == PH_INTEGRATOR_IMPLICITFAST)
+    for (int u = 0; u < m.d.nu; u++)
+      if (bias[3 * u + 2] != 0.0) return true;
+  return false;
+}
+
+// model.rs:430-452 without the contype/conaffinity test (applied per env on the device).
+bool pair_static_ok(const HostModel& m, int g1, int g2) {
+  int b1 = m.geom_body[g1], b2 = m.geom_body[g2];
+  for (int e = 0; e < m.d.nexclude; e++) {
+    int a = m.exclude_pairs[2 * e], b = m.exclude_pairs[2 * e + 1];
+    if ((a == b1 && b == b2) || (a == b2 && b == b1)) return false;
+  }
+  int w1 = m.body_weldid[b1], w2 = m.body_weldid[b2];
+  if (w1 == w2) return false;
+  int p1 = m.body_weldid[m.body_parent[w1]], p2 = m.body_weldid[m.body_parent[w2]];
+  if ((p1 == w2 && w2 != 0) || (p2 == w1 && w1 != 0)) return false;
+  return !(m.geom_type[g1] == PH_GEOM_PLANE && m.geom_type[g2] == PH_GEOM_PLANE);
+}
+
+static bool needs_general(int a, int b) {
+  return a == PH_GEOM_MESH || b == PH_GEOM_MESH || (a == PH_GEOM_CYLINDER && b != PH_GEOM_PLANE) ||
+         (b == PH_GEOM_CYLINDER && a != PH_GEOM_PLANE);
+}
+
+static int pair_max_hits(int a, int b) {
+  if (a > b) std::swap(a, b);
+  if (a == PH_GEOM_PLANE && b == PH_GEOM_CAPSULE) return 2;
+  if ((a == PH_GEOM_PLANE && (b == PH_GEOM_BOX || b == PH_GEOM_CYLINDER)) || (a == PH_GEOM_BOX && b == PH_GEOM_BOX))
+    return 4;
+  if (a == PH_GEOM_CAPSULE && (b == PH_GEOM_CAPSULE || b == PH_GEOM_BOX)) return 3;
+  return 1;
+}
+
+// model.rs:318-363
+static void compute_dof_blocks(HostModel& m) {
+  int nv = m.d.nv;
+  std::vector<int> root(nv), group;
+  for (int k = 0; k < nv; k++) {
+    int r = k;
+    while (m.dof_parent[r] >= 0) r = m.dof_parent[r];
+    root[k] = r;
+  }
+  group = root;
+  auto find = [&](int x) {
+    while (group[x] != x) x = group[x];
+    return x;
+  };
+  for (int u = 0; u < m.d.nu; u++) {
+    int a0 = m.actuator_moment_adr[u], n = m.actuator_moment_num[u];
+    for (int w = 0; w + 1 < n; w++) {
+      int a = find(root[m.jnt_dofadr[m.moment_jnt[a0 + w]]]);
+      int b = find(root[m.jnt_dofadr[m.moment_jnt[a0 + w + 1]]]);
+      if (a != b) group[std::max(a, b)] = std::min(a, b);
+    }
+  }
+  std::vector<int> gid(nv);
+  for (int k = 0; k < nv; k++) gid[k] = find(root[k]);
+  m.block_adr.clear();
+  std::set<int> seen;
+  int k = 0;
+  m.block_adr.push_back(0);
+  while (k < nv) {
+    int e = k + 1;
+    while (e < nv && gid[e] == gid[k]) e++;
+    if (!seen.insert(gid[k]).second) {
+      m.block_adr.assign({0, nv});
+      return;
+    }
+    m.block_adr.push_back(e);
+    k = e;
+  }
+  if (nv == 0) m.block_adr.clear();
+}
+
+bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
+  m.d = s;
+  int nb = s.nbody, nj = s.njnt, nv = s.nv, ng = s.ngeom, nu = s.nu;
+  if (nb < 1) {
+    *err = "model needs a world body";
+    return false;
+  }
+  if (nv > 64) {
+    *err = "models with more than 64 dofs are not supported by the device path";
+    return false;
+  }
+  cp(m.body_parent, s.body_parent, nb);
+  cp(m.body_pos, s.body_pos, 3 * nb);
+  cp(m.body_quat, s.body_quat, 4 * nb);
+  cp(m.body_jntadr, s.body_jntadr, nb);
+  cp(m.body_jntnum, s.body_jntnum, nb);
+  cp(m.body_dofadr, s.body_dofadr, nb);
+  cp(m.body_dofnum, s.body_dofnum, nb);
+  cp(m.body_mass, s.body_mass, nb);
+  cp(m.body_ipos, s.body_ipos, 3 * nb);
+  cp(m.body_inertia, s.body_inertia, 9 * nb);
+  cp(m.jnt_type, s.jnt_type, nj);
+  cp(m.jnt_body, s.jnt_body, nj);
+  cp(m.jnt_qposadr, s.jnt_qposadr, nj);
+  cp(m.jnt_dofadr, s.jnt_dofadr, nj);
+  cp(m.jnt_pos, s.jnt_pos, 3 * nj);
+  cp(m.jnt_axis, s.jnt_axis, 3 * nj);
+  cp(m.jnt_limited, s.jnt_limited, nj);
+  cp(m.jnt_range, s.jnt_range, 2 * nj);
+  cp(m.jnt_stiffness, s.jnt_stiffness, nj);
+  cp(m.jnt_springref, s.jnt_springref, nj);
+  cp(m.dof_body, s.dof_body, nv);
+  cp(m.dof_jnt, s.dof_jnt, nv);
+  cp(m.dof_damping, s.dof_damping, nv);
+  cp(m.dof_armature, s.dof_armature, nv);
+  cp(m.geom_type, s.geom_type, ng);
+  cp(m.geom_body, s.geom_body, ng);
+  cp(m.geom_pos, s.geom_pos, 3 * ng);
+  cp(m.geom_quat, s.geom_quat, 4 * ng);
+  cp(m.geom_size, s.geom_size, 3 * ng);
+  cp(m.geom_friction, s.geom_friction, ng);
+  cp(m.geom_contype, s.geom_contype, ng);
+  cp(m.geom_conaffinity, s.geom_conaffinity, ng);
+  cp(m.geom_rgba, s.geom_rgba, 4 * ng);
+  cp(m.geom_group, s.geom_group, ng);
+  cp(m.geom_dataid, s.geom_dataid, ng);
+  cp(m.cam_body, s.cam_body, s.ncam);
+  cp(m.cam_pos, s.cam_pos, 3 * s.ncam);
+  cp(m.cam_quat, s.cam_quat, 4 * s.ncam);
+  cp(m.cam_fovy, s.cam_fovy, s.ncam);
+  cp(m.actuator_gear, s.actuator_gear, nu);
+  cp(m.actuator_gain, s.actuator_gain, nu);
+  cp(m.actuator_bias, s.actuator_bias, 3 * nu);
+  cp(m.actuator_ctrllimited, s.actuator_ctrllimited, nu);
+  cp(m.actuator_ctrlrange, s.actuator_ctrlrange, 2 * nu);
+  cp(m.actuator_forcelimited, s.actuator_forcelimited, nu);
+  cp(m.actuator_forcerange, s.actuator_forcerange, 2 * nu);
+  cp(m.actuator_moment_adr, s.actuator_moment_adr, nu);
+  cp(m.actuator_moment_num, s.actuator_moment_num, nu);
+  cp(m.moment_jnt, s.moment_jnt, s.nmoment);
+  cp(m.moment_coef, s.moment_coef, s.nmoment);
+  cp(m.eq_joint1, s.eq_joint1, s.neq);
+  cp(m.eq_joint2, s.eq_joint2, s.neq);
+  cp(m.eq_polycoef, s.eq_polycoef, 5 * s.neq);
+  cp(m.eq_solref, s.eq_solref, 2 * s.neq);
+  cp(m.eq_solimp, s.eq_solimp, 5 * s.neq);
+  cp(m.exclude_pairs, s.exclude_pairs, 2 * s.nexclude);
+  cp(m.qpos0, s.qpos0, s.nq);
+  cp(m.qpos_reset, s.qpos_reset, s.nq);
+  if ((int)m.qpos_reset.size() != s.nq) m.qpos_reset = m.qpos0;
+  retarget(m);
+
+  // dof tree (model.rs:285-296)
+  m.body_lastdof.assign(nb, -1);
+  m.dof_parent.assign(nv, -1);
+  for (int b = 1; b < nb; b++) {
+    int last = m.body_lastdof[m.body_parent[b]];
+    for (int k = 0; k < m.body_dofnum[b]; k++) {
+      int d = m.body_dofadr[b] + k;
+      m.dof_parent[d] = last;
+      last = d;
+    }
+    m.body_lastdof[b] = last;
+  }
+  compute_dof_blocks(m);
+  // weld ids (model.rs:301-306)
+  m.body_weldid.resize(nb);
+  for (int b = 0; b < nb; b++) m.body_weldid[b] = b;
+  for (int b = 1; b < nb; b++)
+    if (m.body_jntnum[b] == 0) m.body_weldid[b] = m.body_weldid[m.body_parent[b]];
+  // dof chains
+  m.body_chain.assign(nb, 0);
+  for (int b = 1; b < nb; b++)
+    for (int k = m.body_lastdof[b]; k >= 0; k = m.dof_parent[k]) m.body_chain[b] |= (uint64_t)1 << k;
+  m.dof_anc.assign(nv, 0);
+  for (int k = 0; k < nv; k++)
+    for (int a = m.dof_parent[k]; a >= 0; a = m.dof_parent[a]) m.dof_anc[k] |= (uint64_t)1 << a;
+  // bodies grouped by depth, parents before children
+  std::vector<int> depth(nb, 0);
+  int maxd = 0;
+  for (int b = 1; b < nb; b++) {
+    if (m.body_parent[b] >= b) {
+      *err = "bodies must be listed after their parents";
+      return false;
+    }
+    depth[b] = depth[m.body_parent[b]] + 1;
+    maxd = std::max(maxd, depth[b]);
+  }
+  m.level_adr.clear();
+  m.level_body.clear();
+  for (int l = 1; l <= maxd; l++) {
+    m.level_adr.push_back((int)m.level_body.size());
+    for (int b = 1; b < nb; b++)
+      if (depth[b] == l) m.level_body.push_back(b);
+  }
+  m.level_adr.push_back((int)m.level_body.size());
+  // local boxes
+  m.geom_aabb.assign(6 * ng, 0.0);
+  for (int g = 0; g < ng; g++) local_aabb(m.geom_type[g], &m.geom_size[3 * g], &m.geom_aabb[6 * g]);
+  // candidate pairs in lexicographic order (model.rs:398-407)
+  m.pair_g1.clear();
+  m.pair_g2.clear();
+  m.ncmax = 0;
+  for (int g1 = 0; g1 < ng; g1++)
+    for (int g2 = g1 + 1; g2 < ng; g2++) {
+      if (!pair_static_ok(m, g1, g2)) continue;
+      if (needs_general(m.geom_type[g1], m.geom_type[g2])) {
+        *err = "mesh and cylinder-vs-non-plane pairs need the general convex narrow phase, not implemented on the device";
+        return false;
+      }
+      m.pair_g1.push_back(g1);
+      m.pair_g2.push_back(g2);
+      m.ncmax += pair_max_hits(m.geom_type[g1], m.geom_type[g2]);
+    }
+  // actuator contributions per dof, in (actuator, moment entry) order (forward.rs:207-209)
+  m.dofact_adr.assign(nv, 0);
+  m.dofact_num.assign(nv, 0);
+  m.dofact_u.clear();
+  m.dofact_coef.clear();
+  for (int k = 0; k < nv; k++) {
+    m.dofact_adr[k] = (int)m.dofact_u.size();
+    for (int u = 0; u < nu; u++)
+      for (int i = 0; i < m.actuator_moment_num[u]; i++) {
+        int e = m.actuator_moment_adr[u] + i;
+        if (m.jnt_dofadr[m.moment_jnt[e]] == k) {
+          m.dofact_u.push_back(u);
+          m.dofact_coef.push_back(m.moment_coef[e]);
+        }
+      }
+    m.dofact_num[k] = (int)m.dofact_u.size() - m.dofact_adr[k];
+  }
+  m.nlimited = 0;
+  for (int j = 0; j < nj; j++)
+    if (m.jnt_limited[j] && (m.jnt_type[j] == PH_JNT_HINGE || m.jnt_type[j] == PH_JNT_SLIDE)) m.nlimited++;
+  m.contact_hertz = s.contact_hertz > 0.0 ? s.contact_hertz : 0.25 / s.timestep;
+  m.damped = has_implicit_terms(m, m.dof_damping.data(), m.actuator_bias.data());
+  return true;
+}
+
+// Scalars, capacities and LDS layout of the device model (pointers are left null).
+void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk::DModel& M) {
+  const ph_model_desc& d = hm.d;
+  memset(&M, 0, sizeof(M));
+  M.nq = d.nq;
+  M.nv = d.nv;
+  M.nu = d.nu;
+  M.nbody = d.nbody;
+  M.njnt = d.njnt;
+  M.ngeom = d.ngeom;
+  M.ncam = d.ncam;
+  M.neq = d.neq;
+  M.npair = (int)hm.pair_g1.size();
+  M.nlevel = (int)hm.level_adr.size() - 1;
+  M.nblock = hm.block_adr.empty() ? 0 : (int)hm.block_adr.size() - 1;
+  M.integrator = d.integrator;
+  M.solver_kind = solver;
+  M.iterations = d.solver_iterations;
+  M.timestep = d.timestep;
+  for (int k = 0; k < 3; k++) M.gravity[k] = d.gravity[k];
+  M.tolerance = d.tolerance;
+  M.margin = d.contact_margin;
+  M.solref[0] = d.solref[0];
+  M.solref[1] = d.solref[1];
+  for (int k = 0; k < 5; k++) M.solimp[k] = d.solimp[k];
+  M.impratio = d.impratio;
+  M.hertz = hm.contact_hertz;
+  M.zeta = d.contact_damping_ratio;
+  M.slop = d.contact_slop;
+  M.ncmax = std::max(hm.ncmax, 1);
+  if (max_contacts > 0) M.ncmax = std::min(M.ncmax, max_contacts);
+  M.nrmax = std::max(4 * M.ncmax + 2 * hm.nlimited + d.neq, 1);
+  // keep the LDS footprint modest: rows beyond nrl live in HBM
+  M.nrl = std::min(M.nrmax, 32);
+  M.L = make_layout(hm, M.nrl, M.npair, M.ncmax);
+
+}
+
+// LDS layout of one env (offsets in doubles, then ints).
+phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax) {
+  phk::Layout L;
+  const ph_model_desc& d = hm.d;
+  int nq = d.nq, nv = d.nv, nu = d.nu, nb = d.nbody, ng = d.ngeom;
+  int o = 0;
+  auto take = [&](int n) {
+    int r = o;
+    o += n;
+    return r;
+  };
+  L.qpos = take(nq);
+  L.qvel = take(nv);
+  L.ctrl = take(nu);
+  L.qacc = take(nv);
+  L.applied = take(nv);
+  L.xpos = take(3 * nb);
+  L.xquat = take(4 * nb);
+  L.xmat = take(9 * nb);
+  L.xipos = take(3 * nb);
+  L.cdof = take(6 * nv);
+  L.cdof_dot = take(6 * nv);
+  L.cinert = take(13 * nb);
+  L.crb = take(13 * nb);
+  L.cvel = take(6 * nb);
+  L.cacc = take(6 * nb);
+  L.cfrc = take(6 * nb);
+  L.qm = take(nv * nv);
+  L.chol = take(nv * nv);
+  L.H = take(nv * nv);
+  L.bias = take(nv);
+  L.passive = take(nv);
+  L.qfrc_act = take(nv);
+  L.actf = take(nu);
+  L.actclamp = take(nu);
+  L.smooth = take(nv);
+  L.qfrc_c = take(nv);
+  L.acc = take(nv);
+  L.vel = take(nv);
+  L.gxpos = take(3 * ng);
+  L.gxmat = take(9 * ng);
+  L.aabb = take(6 * ng);
+  L.a = take(nv);
+  L.a0 = take(nv);
+  L.grad = take(nv);
+  L.p = take(nv);
+  L.tmp = take(nv);
+  L.tmp2 = take(nv);
+  L.mp = take(nv);
+  L.J = take(nrl * nv);
+  L.aref = take(nrl);
+  L.dd = take(nrl);
+  L.jar = take(nrl);
+  L.jp = take(nrl);
+  L.rpos = take(nrl);
+  L.rdiag = take(nrl);
+  L.ract = take(nrl);
+  L.rprev = take(nrl);
+  L.rf = take(nrl);
+  L.scal = take(16);
+  L.ndouble = o;
+  int oi = 0;
+  L.i_paircnt = oi;
+  oi += npair;
+  L.i_pairoff = oi;
+  oi += npair;
+  L.i_cact = oi;
+  oi += ncmax;
+  L.i_iscal = oi;
+  oi += 16;
+  L.nint = oi;
+  L.bytes = (int)(L.ndouble * sizeof(double) + L.nint * sizeof(int));
+  return L;
+}
+
+}  // namespace phh

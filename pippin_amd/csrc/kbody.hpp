@@ -1,0 +1,1017 @@
+// The batched physics step, written once against an execution policy X:
+//   X::par(n, f)  runs f(i) for i in [0, n) spread over the lanes of the
+//                 env's wavefront, then makes the writes visible to all lanes;
+//   X::lane()     this lane's index (0 on the serial policy).
+// The device policy maps one environment to one 64-lane wavefront (one
+// workgroup), so every loop whose trip count depends on the env's state
+// (contacts, rows, Newton iterations, line search) is wave-uniform.
+//
+// Stages restate crates/pippin/src/forward.rs::step and its callees. Work is
+// split across lanes only along axes where the reference's floating-point
+// order is per element (bodies of one tree level, spatial-inertia components,
+// dofs, pairs, matrix entries, Cholesky rows), so each lane performs exactly
+// the reference's sequence of operations for the values it owns.
+#pragma once
+#include "dcollide.hpp"
+
+namespace phk {
+
+struct Env {
+  Real* s;  // LDS doubles
+  int* si;  // LDS ints
+  int64_t e;
+  int var;
+  // row storage for the current solve (LDS when it fits, else global)
+  Real *J, *aref, *dd, *jar, *jp, *rpos, *rdiag, *ract, *rprev, *rf;
+};
+
+#define LD(name) (E.s + M.L.name)
+
+template <class X>
+PHD void rows_bind(const DModel& M, const DState& S, Env& E, int nr) {
+  if (nr <= M.nrl) {
+    E.J = LD(J);
+    E.aref = LD(aref);
+    E.dd = LD(dd);
+    E.jar = LD(jar);
+    E.jp = LD(jp);
+    E.rpos = LD(rpos);
+    E.rdiag = LD(rdiag);
+    E.ract = LD(ract);
+    E.rprev = LD(rprev);
+    E.rf = LD(rf);
+  } else {
+    E.J = S.gjac + E.e * (int64_t)M.nrmax * M.nv;
+    Real* g = S.grow + E.e * (int64_t)M.nrmax * 10;
+    E.aref = g;
+    E.dd = g + M.nrmax;
+    E.jar = g + 2 * M.nrmax;
+    E.jp = g + 3 * M.nrmax;
+    E.rpos = g + 4 * M.nrmax;
+    E.rdiag = g + 5 * M.nrmax;
+    E.ract = g + 6 * M.nrmax;
+    E.rprev = g + 7 * M.nrmax;
+    E.rf = g + 8 * M.nrmax;
+  }
+}
+
+// ---------------- kinematics, forward.rs:14-90 ----------------
+template <class X>
+PHD void fk_body(const DModel& M, Env& E, int b) {
+  Real* xpos = LD(xpos);
+  Real* xquat = LD(xquat);
+  Real* xmat = LD(xmat);
+  const Real* q = LD(qpos);
+  Real* cdof = LD(cdof);
+  int p = M.body_parent[b];
+  V3 pos = ld3(xpos + 3 * p) + mul_vec(ldm(xmat + 9 * p), ld3(M.body_pos + 3 * b));
+  Q4 quat = qmul(ldq(xquat + 4 * p), ldq(M.body_quat + 4 * b));
+  int j0 = M.body_jntadr[b], j1 = j0 + M.body_jntnum[b];
+  for (int j = j0; j < j1; j++) {
+    int qa = M.jnt_qposadr[j], da = M.jnt_dofadr[j];
+    int t = M.jnt_type[j];
+    if (t == J_FREE) {
+      pos = v3(q[qa], q[qa + 1], q[qa + 2]);
+      quat = qnormalized(q4(q[qa + 3], q[qa + 4], q[qa + 5], q[qa + 6]));
+      M3 r = to_mat(quat);
+      for (int k = 0; k < 3; k++) {
+        V3 e = v3(k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0);
+        stsp(cdof + 6 * (da + k), sp(v3z(), e));
+        V3 a = col(r, k);
+        stsp(cdof + 6 * (da + 3 + k), sp(a, cross(pos, a)));
+      }
+    } else if (t == J_BALL) {
+      V3 jp = ld3(M.jnt_pos + 3 * j);
+      V3 anchor = pos + qrotate(quat, jp);
+      Q4 ql = qnormalized(q4(q[qa], q[qa + 1], q[qa + 2], q[qa + 3]));
+      quat = qmul(quat, ql);
+      pos = anchor - qrotate(quat, jp);
+      M3 r = to_mat(quat);
+      for (int k = 0; k < 3; k++) {
+        V3 a = col(r, k);
+        stsp(cdof + 6 * (da + k), sp(a, cross(anchor, a)));
+      }
+    } else if (t == J_HINGE) {
+      V3 jp = ld3(M.jnt_pos + 3 * j), jax = ld3(M.jnt_axis + 3 * j);
+      V3 anchor = pos + qrotate(quat, jp);
+      V3 axis = qrotate(quat, jax);
+      quat = qmul(quat, from_axis_angle(jax, q[qa] - M.qpos0[qa]));
+      pos = anchor - qrotate(quat, jp);
+      stsp(cdof + 6 * da, sp(axis, cross(anchor, axis)));
+    } else {
+      V3 jp = ld3(M.jnt_pos + 3 * j), jax = ld3(M.jnt_axis + 3 * j);
+      V3 anchor = pos + qrotate(quat, jp);
+      (void)anchor;
+      V3 axis = qrotate(quat, jax);
+      pos = pos + axis * (q[qa] - M.qpos0[qa]);
+      stsp(cdof + 6 * da, sp(v3z(), axis));
+    }
+  }
+  quat = qnormalized(quat);
+  M3 xm = to_mat(quat);
+  st3(xpos + 3 * b, pos);
+  stq(xquat + 4 * b, quat);
+  stm(xmat + 9 * b, xm);
+  V3 xipos = pos + mul_vec(xm, ld3(M.v_body_ipos + ((int64_t)E.var * M.nbody + b) * 3));
+  st3(LD(xipos) + 3 * b, xipos);
+  // SpatialInertia::from_com(mass, xipos, I.rotate(xmat)), math.rs:388-393
+  Real mass = M.v_body_mass[(int64_t)E.var * M.nbody + b];
+  M3 ic = mrotate(ldm(M.v_body_inertia + ((int64_t)E.var * M.nbody + b) * 9), xm);
+  M3 cx = skew(xipos);
+  M3 io = madd(ic, mscale(mul_mat(cx, transpose(cx)), mass));
+  Real* ci = LD(cinert) + 13 * b;
+  ci[0] = mass;
+  st3(ci + 1, xipos * mass);
+  stm(ci + 4, io);
+}
+
+template <class X>
+PHD void kinematics(const DModel& M, Env& E) {
+  X::par(1, [&](int) {
+    st3(LD(xpos), v3z());
+    stq(LD(xquat), q4(1, 0, 0, 0));
+    stm(LD(xmat), m3id());
+    st3(LD(xipos), v3z());
+    Real* ci = LD(cinert);
+    for (int k = 0; k < 13; k++) ci[k] = 0.0;
+  });
+  for (int l = 0; l < M.nlevel; l++) {
+    int b0 = M.level_adr[l], nb = M.level_adr[l + 1] - b0;
+    X::par(nb, [&](int i) { fk_body<X>(M, E, M.level_body[b0 + i]); });
+  }
+  X::par(M.ngeom, [&](int g) {
+    int b = M.geom_body[g];
+    M3 xm = ldm(LD(xmat) + 9 * b);
+    st3(LD(gxpos) + 3 * g, ld3(LD(xpos) + 3 * b) + mul_vec(xm, ld3(M.geom_pos + 3 * g)));
+    stm(LD(gxmat) + 9 * g, mul_mat(xm, to_mat(ldq(M.geom_quat + 4 * g))));
+  });
+}
+
+// ---------------- CRBA, forward.rs:93-112 ----------------
+template <class X>
+PHD void crba(const DModel& M, Env& E) {
+  const int nb = M.nbody, nv = M.nv;
+  Real* crb = LD(crb);
+  const Real* ci = LD(cinert);
+  Real* qm = LD(qm);
+  // composite inertia leaf to root: each of the 13 components is its own
+  // sequence of additions in the reference's (decreasing body) order
+  X::par(13, [&](int c) {
+    for (int b = 0; b < nb; b++) crb[13 * b + c] = ci[13 * b + c];
+    for (int b = nb - 1; b >= 1; b--) {
+      int p = M.body_parent[b];
+      crb[13 * p + c] = crb[13 * p + c] + crb[13 * b + c];
+    }
+  });
+  X::par(nv * nv, [&](int k) { qm[k] = 0.0; });
+  X::par(nv, [&](int i) {
+    const Real* cd = LD(cdof);
+    SP f = mul_motion(crb + 13 * M.dof_body[i], ldsp(cd + 6 * i));
+    int j = i;
+    while (j >= 0) {
+      Real v = sdot(ldsp(cd + 6 * j), f);
+      qm[i * nv + j] = v;
+      qm[j * nv + i] = v;
+      j = M.dof_parent[j];
+    }
+    qm[i * nv + i] += M.dof_armature[i];
+  });
+}
+
+// ---------------- velocity, forward.rs:115-144 ----------------
+template <class X>
+PHD void velocity(const DModel& M, Env& E) {
+  Real* cvel = LD(cvel);
+  Real* cdd = LD(cdof_dot);
+  const Real* cd = LD(cdof);
+  const Real* qvel = LD(qvel);
+  X::par(1, [&](int) { stsp(cvel, sp(v3z(), v3z())); });
+  for (int l = 0; l < M.nlevel; l++) {
+    int b0 = M.level_adr[l], nb = M.level_adr[l + 1] - b0;
+    X::par(nb, [&](int i) {
+      int b = M.level_body[b0 + i];
+      SP v = ldsp(cvel + 6 * M.body_parent[b]);
+      int j0 = M.body_jntadr[b], j1 = j0 + M.body_jntnum[b];
+      for (int j = j0; j < j1; j++) {
+        int da = M.jnt_dofadr[j], t = M.jnt_type[j];
+        if (t == J_FREE || t == J_BALL) {
+          int lin = t == J_FREE ? 3 : 0, rot = t == J_FREE ? da + 3 : da;
+          for (int k = 0; k < lin; k++) {
+            stsp(cdd + 6 * (da + k), sp(v3z(), v3z()));
+            v = spadd(v, sscale(ldsp(cd + 6 * (da + k)), qvel[da + k]));
+          }
+          for (int k = 0; k < 3; k++) stsp(cdd + 6 * (rot + k), cross_motion(v, ldsp(cd + 6 * (rot + k))));
+          for (int k = 0; k < 3; k++) v = spadd(v, sscale(ldsp(cd + 6 * (rot + k)), qvel[rot + k]));
+        } else {
+          stsp(cdd + 6 * da, cross_motion(v, ldsp(cd + 6 * da)));
+          v = spadd(v, sscale(ldsp(cd + 6 * da), qvel[da]));
+        }
+      }
+      stsp(cvel + 6 * b, v);
+    });
+  }
+}
+
+// ---------------- RNE, forward.rs:147-171 ----------------
+template <class X>
+PHD void rne(const DModel& M, Env& E) {
+  Real* cacc = LD(cacc);
+  Real* cfrc = LD(cfrc);
+  const Real* cvel = LD(cvel);
+  const Real* cdd = LD(cdof_dot);
+  const Real* ci = LD(cinert);
+  const Real* qvel = LD(qvel);
+  X::par(1, [&](int) {
+    stsp(cacc, sp(v3z(), -v3(M.gravity[0], M.gravity[1], M.gravity[2])));
+    stsp(cfrc, sp(v3z(), v3z()));
+  });
+  for (int l = 0; l < M.nlevel; l++) {
+    int b0 = M.level_adr[l], nb = M.level_adr[l + 1] - b0;
+    X::par(nb, [&](int i) {
+      int b = M.level_body[b0 + i];
+      SP a = ldsp(cacc + 6 * M.body_parent[b]);
+      int k0 = M.body_dofadr[b], k1 = k0 + M.body_dofnum[b];
+      for (int k = k0; k < k1; k++) a = spadd(a, sscale(ldsp(cdd + 6 * k), qvel[k]));
+      stsp(cacc + 6 * b, a);
+      SP v = ldsp(cvel + 6 * b);
+      SP h = mul_motion(ci + 13 * b, v);
+      stsp(cfrc + 6 * b, spadd(mul_motion(ci + 13 * b, a), cross_force(v, h)));
+    });
+  }
+  X::par(6, [&](int c) {
+    for (int b = M.nbody - 1; b >= 1; b--) {
+      int p = M.body_parent[b];
+      cfrc[6 * p + c] = cfrc[6 * p + c] + cfrc[6 * b + c];
+    }
+  });
+  X::par(M.nv, [&](int k) {
+    LD(bias)[k] = sdot(ldsp(LD(cdof) + 6 * k), ldsp(cfrc + 6 * M.dof_body[k]));
+  });
+}
+
+// ---------------- passive + actuation, forward.rs:173-211 ----------------
+template <class X>
+PHD void passive_actuation(const DModel& M, Env& E) {
+  const Real* qpos = LD(qpos);
+  const Real* qvel = LD(qvel);
+  X::par(M.nv, [&](int k) {
+    Real f = -M.v_dof_damping[(int64_t)E.var * M.nv + k] * qvel[k];
+    int j = M.dof_jnt[k];
+    Real kp = M.jnt_stiffness[j];
+    int t = M.jnt_type[j];
+    if (kp != 0.0 && (t == J_HINGE || t == J_SLIDE)) f -= kp * (qpos[M.jnt_qposadr[j]] - M.jnt_springref[j]);
+    LD(passive)[k] = f;
+  });
+  X::par(M.nu, [&](int u) {
+    Real gear = M.act_gear[u];
+    int a0 = M.act_madr[u], a1 = a0 + M.act_mnum[u];
+    Real sl = rsum0(), sv = rsum0();
+    for (int i = a0; i < a1; i++) sl += M.mom_coef[i] * qpos[M.jnt_qposadr[M.mom_jnt[i]]];
+    for (int i = a0; i < a1; i++) sv += M.mom_coef[i] * qvel[M.jnt_dofadr[M.mom_jnt[i]]];
+    Real length = gear * sl, vel = gear * sv;
+    Real ctrl = LD(ctrl)[u];
+    if (M.act_ctrllimited[u]) ctrl = rclamp(ctrl, M.act_ctrlrange[2 * u], M.act_ctrlrange[2 * u + 1]);
+    const Real* b = M.v_act_bias + ((int64_t)E.var * M.nu + u) * 3;
+    Real f = M.v_act_gain[(int64_t)E.var * M.nu + u] * ctrl + b[0] + b[1] * length + b[2] * vel;
+    Real clamped = 0.0;
+    if (M.act_forcelimited[u]) {
+      Real lo = M.act_forcerange[2 * u], hi = M.act_forcerange[2 * u + 1];
+      clamped = (f < lo || f > hi) ? 1.0 : 0.0;
+      f = rclamp(f, lo, hi);
+    }
+    LD(actf)[u] = f;
+    LD(actclamp)[u] = clamped;
+  });
+  X::par(M.nv, [&](int k) {
+    Real s = 0.0;
+    int a0 = M.dofact_adr[k], a1 = a0 + M.dofact_num[k];
+    for (int i = a0; i < a1; i++) {
+      int u = M.dofact_u[i];
+      s += M.act_gear[u] * M.dofact_coef[i] * LD(actf)[u];
+    }
+    LD(qfrc_act)[k] = s;
+  });
+}
+
+// ---------------- collision, forward.rs:214-260 ----------------
+template <class X>
+PHD void collide(const DModel& M, const DState& S, Env& E) {
+  const int ng = M.ngeom;
+  Real* aabb = LD(aabb);
+  const Real margin = M.margin;
+  X::par(ng, [&](int g) {
+    const Real* la = M.v_geom_aabb + ((int64_t)E.var * ng + g) * 6;
+    V3 c = ld3(la), h = ld3(la + 3);
+    const Real* r = LD(gxmat) + 9 * g;
+    if (!rfinite(h.x)) {
+      st3(aabb + 6 * g, v3z());
+      st3(aabb + 6 * g + 3, h);
+      return;
+    }
+    M3 rm = ldm(r);
+    st3(aabb + 6 * g, ld3(LD(gxpos) + 3 * g) + mul_vec(rm, c));
+    st3(aabb + 6 * g + 3,
+        v3(fabs(r[0]) * h.x + fabs(r[1]) * h.y + fabs(r[2]) * h.z, fabs(r[3]) * h.x + fabs(r[4]) * h.y + fabs(r[5]) * h.z,
+           fabs(r[6]) * h.x + fabs(r[7]) * h.y + fabs(r[8]) * h.z));
+  });
+  int* cnt = E.si + M.L.i_paircnt;
+  Real* stage = S.stage + E.e * (int64_t)M.npair * MAXHIT * CW;
+  X::par(M.npair, [&](int p) {
+    int g1 = M.pair_g1[p], g2 = M.pair_g2[p];
+    cnt[p] = 0;
+    const uint32_t* ct = M.v_contype + (int64_t)E.var * ng;
+    const uint32_t* ca = M.v_conaffinity + (int64_t)E.var * ng;
+    if ((ct[g1] & ca[g2]) == 0 && (ct[g2] & ca[g1]) == 0) return;  // per-env masks, model.rs:439-443
+    const Real* A = aabb + 6 * g1;
+    const Real* B = aabb + 6 * g2;
+    for (int k = 0; k < 3; k++)
+      if (fabs(A[k] - B[k]) > A[3 + k] + B[3 + k] + margin) return;
+    GPose a, b;
+    a.typ = M.geom_type[g1];
+    a.pos = ld3(LD(gxpos) + 3 * g1);
+    a.mat = ldm(LD(gxmat) + 9 * g1);
+    a.size = ld3(M.v_geom_size + ((int64_t)E.var * ng + g1) * 3);
+    b.typ = M.geom_type[g2];
+    b.pos = ld3(LD(gxpos) + 3 * g2);
+    b.mat = ldm(LD(gxmat) + 9 * g2);
+    b.size = ld3(M.v_geom_size + ((int64_t)E.var * ng + g2) * 3);
+    Sink sink;
+    sink.out = stage + (int64_t)p * MAXHIT * CW;
+    sink.n = 0;
+    collide_pair(a, b, margin, sink);
+    cnt[p] = sink.n;
+  });
+  // pair-ordered compaction (contacts are listed in pair order, then hit order)
+  int* off = E.si + M.L.i_pairoff;
+  int* isc = E.si + M.L.i_iscal;
+  X::par(1, [&](int) {
+    int o = 0;
+    for (int p = 0; p < M.npair; p++) {
+      off[p] = o;
+      o += cnt[p];
+    }
+    isc[0] = o;  // ncon
+  });
+  Real* con = S.con + E.e * (int64_t)M.ncmax * CW;
+  int* cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
+  X::par(M.npair, [&](int p) {
+    int n = cnt[p];
+    for (int h = 0; h < n; h++) {
+      const Real* src = stage + ((int64_t)p * MAXHIT + h) * CW;
+      Real* dst = con + (int64_t)(off[p] + h) * CW;
+      for (int k = 0; k < CW; k++) dst[k] = src[k];
+      int* gd = cg + (off[p] + h) * 3;
+      gd[0] = M.pair_g1[p];
+      gd[1] = M.pair_g2[p];
+      gd[2] = h;
+    }
+  });
+}
+
+// Contact::frame, collision.rs:47-55
+PHD void contact_frame(const Real* c, V3* t1, V3* t2) {
+  V3 n = ld3(c + 3), tg = ld3(c + 6);
+  V3 t = tg - n * dot(n, tg);
+  if (norm2(t) > 1e-12) {
+    *t1 = normalized(t);
+    *t2 = cross(n, *t1);
+  } else {
+    tangents(n, t1, t2);
+  }
+}
+
+// newton.rs:52-66
+PHD Real impedance(const Real* solimp, Real pos) {
+  Real dmin = solimp[0], dmax = solimp[1], width = solimp[2], mid = solimp[3], power = solimp[4];
+  Real x = rmin(fabs(pos) / rmax(width, 1e-12), 1.0);
+  if (x >= 1.0 || dmin == dmax) return dmax;
+  Real y;
+  if (x <= mid) {
+    Real num = power == 2.0 ? x * x : (power == 1.0 ? x : pow(x, power));
+    Real pm = power - 1.0;
+    Real den = pm == 2.0 ? mid * mid : (pm == 1.0 ? mid : pow(mid, pm));
+    y = num / den;
+  } else {
+    Real b = 1.0 - x;
+    Real num = power == 2.0 ? b * b : (power == 1.0 ? b : pow(b, power));
+    Real bm = 1.0 - mid, pm = power - 1.0;
+    Real den = pm == 2.0 ? bm * bm : (pm == 1.0 ? bm : pow(bm, pm));
+    y = 1.0 - num / den;
+  }
+  return rclamp(dmin + y * (dmax - dmin), 1e-4, 0.9999);
+}
+
+// ---------------- block Cholesky, math.rs:425-463 ----------------
+// Factor the diagonal block [s0, e) of the nv x nv row-major matrix A in place.
+// Column j: the diagonal (one lane), then the entries below it (one lane per
+// row, each the reference's 4-accumulator dot). Returns false if not PD.
+template <class X>
+PHD bool chol_block(const DModel& M, Env& E, Real* A, int nv, int s0, int e) {
+  int* isc = E.si + M.L.i_iscal;
+  for (int j = s0; j < e; j++) {
+    X::par(1, [&](int) {
+      Real* rj = A + j * nv + s0;
+      Real d = A[j * nv + j] - sdot_n(rj, rj, j - s0);
+      if (d <= 0.0) {
+        isc[1] = 1;
+        return;
+      }
+      A[j * nv + j] = sqrt(d);
+    });
+    if (isc[1]) return false;
+    X::par(e - j - 1, [&](int t) {
+      int i = j + 1 + t;
+      Real inv = 1.0 / A[j * nv + j];
+      Real* ri = A + i * nv + s0;
+      int k = j - s0;
+      ri[k] = (ri[k] - sdot_n(ri, A + j * nv + s0, k)) * inv;
+    });
+  }
+  return true;
+}
+
+// Forward then backward substitution on block [s0, e); x is an LDS vector.
+template <class X>
+PHD void chol_solve_block(const DModel& M, Env& E, const Real* L, int nv, int s0, int e, Real* x) {
+  (void)M;
+  (void)E;
+  X::par(1, [&](int) {
+    for (int i = s0; i < e; i++) x[i] = (x[i] - sdot_n(L + i * nv + s0, x + s0, i - s0)) / L[i * nv + i];
+    for (int i = e - 1; i >= s0; i--) {
+      x[i] /= L[i * nv + i];
+      Real xi = x[i];
+      const Real* li = L + i * nv + s0;
+      for (int k = 0; k < i - s0; k++) x[s0 + k] += -xi * li[k];
+    }
+  });
+}
+
+template <class X>
+PHD bool factor_mass(const DModel& M, Env& E, Real h) {
+  const int nv = M.nv;
+  Real* chol = LD(chol);
+  const Real* qm = LD(qm);
+  X::par(nv * nv, [&](int k) { chol[k] = qm[k]; });
+  // add_implicit_terms, forward.rs:407-431
+  X::par(nv, [&](int k) { chol[k * nv + k] += h * M.v_dof_damping[(int64_t)E.var * nv + k]; });
+  if (M.integrator == 1) {
+    X::par(1, [&](int) {
+      for (int u = 0; u < M.nu; u++) {
+        Real kv = -M.v_act_bias[((int64_t)E.var * M.nu + u) * 3 + 2] * M.act_gear[u] * M.act_gear[u];
+        if (kv == 0.0 || LD(actclamp)[u] != 0.0) continue;
+        int a0 = M.act_madr[u], a1 = a0 + M.act_mnum[u];
+        for (int ia = a0; ia < a1; ia++)
+          for (int ib = a0; ib < a1; ib++) {
+            int a = M.jnt_dofadr[M.mom_jnt[ia]], b = M.jnt_dofadr[M.mom_jnt[ib]];
+            bool anc = ((M.dof_anc[b] >> a) & 1) || ((M.dof_anc[a] >> b) & 1);
+            if (a == b || anc) chol[a * nv + b] += h * kv * M.mom_coef[ia] * M.mom_coef[ib];
+          }
+      }
+    });
+  }
+  for (int blk = 0; blk < M.nblock; blk++)
+    if (!chol_block<X>(M, E, chol, nv, M.block_adr[blk], M.block_adr[blk + 1])) return false;
+  return true;
+}
+
+template <class X>
+PHD void mass_solve(const DModel& M, Env& E, Real* x) {
+  for (int blk = 0; blk < M.nblock; blk++)
+    chol_solve_block<X>(M, E, LD(chol), M.nv, M.block_adr[blk], M.block_adr[blk + 1], x);
+}
+
+// ---------------- constraint rows, newton.rs:113-218 ----------------
+// Returns nr; rows are bound to LDS or global storage by their count.
+template <class X>
+PHD int build_rows(const DModel& M, const DState& S, Env& E) {
+  const int nv = M.nv;
+  int* isc = E.si + M.L.i_iscal;
+  int* cact = E.si + M.L.i_cact;
+  const Real* con = S.con + E.e * (int64_t)M.ncmax * CW;
+  const int* cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
+  const Real* qpos = LD(qpos);
+  // list of touching contacts, then limit and equality counts
+  X::par(1, [&](int) {
+    int ncon = isc[0], na = 0;
+    for (int c = 0; c < ncon; c++)
+      if (con[c * CW + 9] > 0.0) cact[na++] = c;
+    int nl = 0;
+    for (int j = 0; j < M.njnt; j++) {
+      int t = M.jnt_type[j];
+      if (!M.jnt_limited[j] || !(t == J_HINGE || t == J_SLIDE)) continue;
+      Real q = qpos[M.jnt_qposadr[j]];
+      if (q - M.jnt_range[2 * j] < 0.0) nl++;
+      if (M.jnt_range[2 * j + 1] - q < 0.0) nl++;
+    }
+    isc[2] = na;
+    isc[3] = 4 * na + nl + M.neq;
+    isc[4] = nl;
+  });
+  const int na = isc[2], nr = isc[3];
+  if (nr > M.nrmax) {
+    X::par(1, [&](int) { isc[5] = 1; });
+    return -1;
+  }
+  rows_bind<X>(M, S, E, nr);
+  if (nr == 0) return 0;
+  Real* J = E.J;
+  // contact rows: lane per (contact, dof) builds J_n, J_t1, J_t2 at that dof
+  X::par(na * nv, [&](int t) {
+    int k = t / nv, dof = t % nv;
+    int c = cact[k];
+    const Real* cc = con + c * CW;
+    int g1 = cg[3 * c], g2 = cg[3 * c + 1];
+    int b1 = M.geom_body[g1], b2 = M.geom_body[g2];
+    Real mu = rmax(M.v_friction[(int64_t)E.var * M.ngeom + g1], M.v_friction[(int64_t)E.var * M.ngeom + g2]);
+    V3 t1, t2;
+    contact_frame(cc, &t1, &t2);
+    V3 p = ld3(cc);
+    V3 dirs[3] = {ld3(cc + 3), t1, t2};
+    Real jv[3];
+    SP cd = ldsp(LD(cdof) + 6 * dof);
+    bool on2 = (M.body_chain[b2] >> dof) & 1, on1 = (M.body_chain[b1] >> dof) & 1;
+    for (int r = 0; r < 3; r++) {
+      Real v = 0.0;
+      if (on2) v += 1.0 * dot(dirs[r], point_velocity(cd, p));
+      if (on1) v += -1.0 * dot(dirs[r], point_velocity(cd, p));
+      jv[r] = v;
+    }
+    for (int ti = 0; ti < 2; ti++)
+      for (int si = 0; si < 2; si++) {
+        Real s = si == 0 ? 1.0 : -1.0;
+        J[(4 * k + 2 * ti + si) * nv + dof] = jv[0] + s * mu * jv[1 + ti];
+      }
+  });
+  X::par(na, [&](int k) {
+    int c = cact[k];
+    int g1 = cg[3 * c], g2 = cg[3 * c + 1];
+    int b1 = M.geom_body[g1], b2 = M.geom_body[g2];
+    Real mu = rmax(M.v_friction[(int64_t)E.var * M.ngeom + g1], M.v_friction[(int64_t)E.var * M.ngeom + g2]);
+    const Real* iw = M.v_body_invweight + (int64_t)E.var * M.nbody * 2;
+    Real tran = iw[2 * b1] + iw[2 * b2];
+    Real dpyr = 2.0 * mu * mu * (1.0 + mu * mu) * tran / M.impratio;
+    for (int r = 0; r < 4; r++) {
+      E.rpos[4 * k + r] = -con[c * CW + 9];
+      E.rdiag[4 * k + r] = dpyr;
+    }
+  });
+  // joint limits then equalities, in joint order (serial: few rows)
+  X::par(1, [&](int) {
+    int r = 4 * na;
+    const Real* iw = M.v_dof_invweight + (int64_t)E.var * nv;
+    for (int j = 0; j < M.njnt; j++) {
+      int t = M.jnt_type[j];
+      if (!M.jnt_limited[j] || !(t == J_HINGE || t == J_SLIDE)) continue;
+      Real q = qpos[M.jnt_qposadr[j]];
+      int da = M.jnt_dofadr[j];
+      Real seps[2] = {q - M.jnt_range[2 * j], M.jnt_range[2 * j + 1] - q};
+      for (int s = 0; s < 2; s++) {
+        if (seps[s] < 0.0) {
+          for (int k = 0; k < nv; k++) J[r * nv + k] = 0.0;
+          J[r * nv + da] = s == 0 ? 1.0 : -1.0;
+          E.rpos[r] = seps[s];
+          E.rdiag[r] = iw[da];
+          r++;
+        }
+      }
+    }
+    for (int e = 0; e < M.neq; e++) {
+      int j1 = M.eq_j1[e], j2 = M.eq_j2[e];
+      const Real* c = M.eq_poly + 5 * e;
+      Real q1 = qpos[M.jnt_qposadr[j1]] - M.qpos0[M.jnt_qposadr[j1]];
+      Real poly = c[0], dpoly = 0.0;
+      if (j2 >= 0) {
+        Real x = qpos[M.jnt_qposadr[j2]] - M.qpos0[M.jnt_qposadr[j2]];
+        poly = c[0] + x * (c[1] + x * (c[2] + x * (c[3] + x * c[4])));
+        dpoly = c[1] + x * (2.0 * c[2] + x * (3.0 * c[3] + x * 4.0 * c[4]));
+      }
+      for (int k = 0; k < nv; k++) J[r * nv + k] = 0.0;
+      J[r * nv + M.jnt_dofadr[j1]] += 1.0;
+      Real dg = iw[M.jnt_dofadr[j1]];
+      if (j2 >= 0) {
+        J[r * nv + M.jnt_dofadr[j2]] -= dpoly;
+        dg += iw[M.jnt_dofadr[j2]];
+      }
+      E.rpos[r] = q1 - poly;
+      E.rdiag[r] = dg;
+      r++;
+    }
+  });
+  // reference acceleration and regularisation per row
+  const int n_ineq = nr - M.neq;
+  X::par(nr, [&](int r) {
+    bool bil = r >= n_ineq;
+    const Real* solref = bil ? M.eq_solref + 2 * (r - n_ineq) : M.solref;
+    const Real* solimp = bil ? M.eq_solimp + 5 * (r - n_ineq) : M.solimp;
+    Real tc = rmax(solref[0], 2.0 * M.timestep), dr = solref[1];
+    Real dmax = solimp[1];
+    Real b = 2.0 / (dmax * tc);
+    Real k = 1.0 / (dmax * dmax * tc * tc * dr * dr);
+    Real vel = rsum0();
+    const Real* jr = J + r * nv;
+    const Real* qvel = LD(qvel);
+    for (int i = 0; i < nv; i++) vel += jr[i] * qvel[i];
+    Real imp = impedance(solimp, E.rpos[r]);
+    Real reg = rmax((1.0 - imp) / imp * E.rdiag[r], 1e-15);
+    E.aref[r] = -b * vel - k * imp * E.rpos[r];
+    E.dd[r] = 1.0 / reg;
+  });
+  return nr;
+}
+
+// ---------------- Newton, newton.rs:224-371 (single island) ----------------
+template <class X>
+PHD void mul_m(const DModel& M, Env& E, const Real* x, Real* y) {
+  const int nv = M.nv;
+  const Real* qm = LD(qm);
+  X::par(nv, [&](int i) { y[i] = sdot_n(qm + i * nv, x, nv) + 0.0 * x[i]; });
+}
+
+template <class X>
+PHD int newton_dense(const DModel& M, Env& E, int nr) {
+  const int nv = M.nv;
+  Real* a = LD(a);
+  Real* a0 = LD(a0);
+  Real* grad = LD(grad);
+  Real* p = LD(p);
+  Real* tmp = LD(tmp);
+  Real* tmp2 = LD(tmp2);
+  Real* mp = LD(mp);
+  Real* H = LD(H);
+  Real* sc = LD(scal);
+  int* isc = E.si + M.L.i_iscal;
+  const Real* qm = LD(qm);
+  const Real* J = E.J;
+  Real* jar = E.jar;
+  Real* jp = E.jp;
+  Real* act = E.ract;
+  Real* prev = E.rprev;
+  X::par(nv, [&](int k) { a0[k] = a[k]; });
+  const int nb = (nr + 63) / 64;
+  // cost(x) = 1/2 (x-a0)'M(x-a0) + sum_active 1/2 D r^2 with r = J x - aref
+  auto cost_into = [&](const Real* x, int slot) {
+    X::par(nv, [&](int k) { tmp[k] = x[k] - a0[k]; });
+    mul_m<X>(M, E, tmp, tmp2);
+    X::par(nr, [&](int r) { jar[r] = sdot_n(J + r * nv, x, nv) - E.aref[r]; });
+    X::par(1, [&](int) {
+      Real s = rsum0();
+      for (int k = 0; k < nv; k++) s += tmp[k] * tmp2[k];
+      Real c = 0.5 * s;
+      for (int r = 0; r < nr; r++) {
+        bool on = r >= nr - M.neq || jar[r] < 0.0;
+        if (on) c += 0.5 * E.dd[r] * jar[r] * jar[r];
+      }
+      sc[slot] = c;
+    });
+  };
+  // warm start from the previous acceleration when cheaper (newton.rs:238-245)
+  const Real* warm = LD(qacc);
+  cost_into(a0, 0);
+  cost_into(warm, 1);
+  if (sc[1] < sc[0]) X::par(nv, [&](int k) { a[k] = warm[k]; });
+  mul_m<X>(M, E, a0, tmp);
+  X::par(1, [&](int) {
+    Real s = rsum0();
+    for (int k = 0; k < nv; k++) s += tmp[k] * tmp[k];
+    sc[2] = 1.0 + sqrt(s);  // scale
+  });
+  const Real scale = sc[2];
+  const Real tol = M.tolerance;
+  const int cap = M.iterations > 1 ? M.iterations : 1;
+  const int n_ineq = nr - M.neq;
+  X::par(nr, [&](int r) { prev[r] = -1.0; });  // u128::MAX sentinel
+  int iters = 0;
+  while (iters < cap) {
+    iters++;
+    X::par(nv, [&](int k) { tmp[k] = a[k] - a0[k]; });
+    mul_m<X>(M, E, tmp, grad);
+    X::par(nr, [&](int r) {
+      Real v = sdot_n(J + r * nv, a, nv) - E.aref[r];
+      jar[r] = v;
+      bool on = r >= n_ineq || v < 0.0;
+      act[r] = on ? 1.0 : 0.0;
+      E.rf[r] = E.dd[r] * v;
+    });
+    X::par(nv, [&](int k) {
+      Real g = grad[k];
+      for (int r = 0; r < nr; r++)
+        if (act[r] != 0.0) g += E.rf[r] * J[r * nv + k];
+      grad[k] = g;
+    });
+    X::par(1, [&](int) {
+      Real s = rsum0();
+      for (int k = 0; k < nv; k++) s += grad[k] * grad[k];
+      Real gnorm = sqrt(s);
+      bool same = nr <= 128 && iters > 1;
+      if (same)
+        for (int r = 0; r < nr && r < 128; r++)
+          if (act[r] != prev[r]) {
+            same = false;
+            break;
+          }
+      isc[6] = (gnorm < tol * scale || same) ? 1 : 0;
+    });
+    if (isc[6]) break;
+    X::par(nr, [&](int r) { prev[r] = act[r]; });
+    // H = M + J_A' D J_A, entry-parallel, rows in order
+    X::par(nv * nv, [&](int t) {
+      int i = t / nv, j = t % nv;
+      Real h = qm[t];
+      if (i == j) h += 0.0;
+      for (int r = 0; r < nr; r++) {
+        if (act[r] == 0.0) continue;
+        Real ji = J[r * nv + i];
+        if (ji == 0.0) continue;
+        h += (E.dd[r] * ji) * J[r * nv + j];
+      }
+      H[t] = h;
+    });
+    X::par(1, [&](int) { isc[1] = 0; });
+    if (!chol_block<X>(M, E, H, nv, 0, nv)) break;
+    X::par(nv, [&](int k) { p[k] = -grad[k]; });
+    chol_solve_block<X>(M, E, H, nv, 0, nv, p);
+    // exact line search along p (newton.rs:310-351)
+    mul_m<X>(M, E, p, mp);
+    X::par(nr, [&](int r) { jp[r] = sdot_n(J + r * nv, p, nv); });
+    X::par(1, [&](int) {
+      Real mpp = rsum0(), mpg = rsum0();
+      for (int k = 0; k < nv; k++) mpp += p[k] * mp[k];
+      for (int k = 0; k < nv; k++) mpg += tmp[k] * mp[k];
+      auto deriv = [&](Real al, Real* d1o, Real* d2o) {
+        Real d1 = al * mpp + mpg, d2 = mpp;
+        for (int r = 0; r < nr; r++) {
+          Real v = jar[r] + al * jp[r];
+          if (r >= n_ineq || v < 0.0) {
+            Real dq = E.dd[r] * jp[r];
+            d1 += dq * v;
+            d2 += dq * jp[r];
+          }
+        }
+        *d1o = d1;
+        *d2o = d2;
+      };
+      Real lo = 0.0, hi = INFINITY, al = 1.0, d1, d2;
+      deriv(0.0, &d1, &d2);
+      Real ltol = 1e-12 * rmax(fabs(d1), 1e-30);
+      for (int it = 0; it < 50; it++) {
+        deriv(al, &d1, &d2);
+        if (fabs(d1) <= ltol) break;
+        if (d1 < 0.0) lo = al;
+        else hi = al;
+        Real next = al - d1 / rmax(d2, 1e-30);
+        if (next <= lo || next >= hi) next = rfinite(hi) ? 0.5 * (lo + hi) : 2.0 * rmax(al, lo);
+        if (fabs(next - al) <= 1e-14 * rmax(fabs(al), 1.0)) {
+          al = next;
+          break;
+        }
+        al = next;
+      }
+      sc[3] = al;
+    });
+    const Real al = sc[3];
+    X::par(nv, [&](int k) { a[k] += al * p[k]; });
+  }
+  (void)nb;
+  // add_constraint_force (newton.rs:361-371)
+  X::par(nr, [&](int r) {
+    Real v = sdot_n(J + r * nv, a, nv) - E.aref[r];
+    act[r] = (r >= n_ineq || v < 0.0) ? 1.0 : 0.0;
+    E.rf[r] = -E.dd[r] * v;
+  });
+  X::par(nv, [&](int k) {
+    Real f = 0.0;
+    for (int r = 0; r < nr; r++)
+      if (act[r] != 0.0) f += E.rf[r] * J[r * nv + k];
+    LD(qfrc_c)[k] = f;
+  });
+  return iters;
+}
+
+// ---------------- integrate_pos, forward.rs:459-480 ----------------
+template <class X>
+PHD void integrate_pos(const DModel& M, Env& E, const Real* v, Real h) {
+  Real* qpos = LD(qpos);
+  X::par(M.njnt, [&](int j) {
+    int qa = M.jnt_qposadr[j], da = M.jnt_dofadr[j], t = M.jnt_type[j];
+    if (t == J_FREE) {
+      for (int k = 0; k < 3; k++) qpos[qa + k] += h * v[da + k];
+      Q4 q = ldq(qpos + qa + 3);
+      Q4 r = integrate_local(q, v3(v[da + 3], v[da + 4], v[da + 5]), h);
+      stq(qpos + qa + 3, r);
+    } else if (t == J_BALL) {
+      Q4 r = integrate_local(ldq(qpos + qa), v3(v[da], v[da + 1], v[da + 2]), h);
+      stq(qpos + qa, r);
+    } else {
+      qpos[qa] += h * v[da];
+    }
+  });
+}
+
+// ---------------- one step, forward.rs:336-389 (Newton) ----------------
+// Returns false when the env cannot be stepped (status set).
+template <class X>
+PHD bool step_env(const DModel& M, const DState& S, Env& E) {
+  const int nv = M.nv;
+  const Real h = M.timestep;
+  int* isc = E.si + M.L.i_iscal;
+  kinematics<X>(M, E);
+  crba<X>(M, E);
+  velocity<X>(M, E);
+  rne<X>(M, E);
+  passive_actuation<X>(M, E);
+  collide<X>(M, S, E);
+  Real* smooth = LD(smooth);
+  Real* acc = LD(acc);
+  Real* a = LD(a);
+  X::par(nv, [&](int k) {
+    Real s = LD(passive)[k] + LD(qfrc_act)[k] + LD(applied)[k] - LD(bias)[k];
+    smooth[k] = s;
+    a[k] = s;
+  });
+  X::par(1, [&](int) {
+    isc[1] = 0;
+    isc[5] = 0;
+  });
+  if (!factor_mass<X>(M, E, 0.0)) {
+    X::par(1, [&](int) { isc[7] |= ST_NOTPD; });
+    return false;
+  }
+  mass_solve<X>(M, E, a);
+  X::par(nv, [&](int k) { LD(qfrc_c)[k] = 0.0; });
+  int nr = build_rows<X>(M, S, E);
+  if (nr < 0) {
+    X::par(1, [&](int) { isc[7] |= ST_ROW_OVERFLOW; });
+    return false;
+  }
+  X::par(1, [&](int) { isc[9] = nr; });
+  if (nr > 0) {
+    int it = newton_dense<X>(M, E, nr);
+    X::par(1, [&](int) { isc[8] = it; });
+  }
+  const bool damped = M.v_damped[E.var] != 0;
+  if (damped) {
+    X::par(1, [&](int) { isc[1] = 0; });
+    if (!factor_mass<X>(M, E, h)) {
+      X::par(1, [&](int) { isc[7] |= ST_NOTPD; });
+      return false;
+    }
+    X::par(nv, [&](int k) { acc[k] = smooth[k] + LD(qfrc_c)[k]; });
+    mass_solve<X>(M, E, acc);
+  } else {
+    X::par(nv, [&](int k) { acc[k] = a[k]; });
+  }
+  Real* vel = LD(vel);
+  X::par(nv, [&](int k) {
+    Real v = LD(qvel)[k] + h * acc[k];
+    vel[k] = v;
+    LD(qacc)[k] = (v - LD(qvel)[k]) / h;
+  });
+  integrate_pos<X>(M, E, vel, h);
+  X::par(nv, [&](int k) { LD(qvel)[k] = vel[k]; });
+  return true;
+}
+
+// ---------------- set_const, forward.rs:275-333 ----------------
+// Inverse inertia at the reset configuration (the caller loads qpos_reset
+// into LDS qpos): dof_invweight (nv) and body_invweight (nbody x 2). Runs the
+// solves on one lane; it is called once per parameter variant, not per step.
+template <class X>
+PHD void set_const_env(const DModel& M, Env& E, Real* dof_iw, Real* body_iw) {
+  const int nv = M.nv, nb = M.nbody;
+  kinematics<X>(M, E);
+  crba<X>(M, E);
+  Real* L = LD(chol);
+  const Real* qm = LD(qm);
+  X::par(nv, [&](int k) { dof_iw[k] = 0.0; });
+  X::par(nb, [&](int b) {
+    body_iw[2 * b] = 0.0;
+    body_iw[2 * b + 1] = 0.0;
+  });
+  if (nv == 0) return;
+  X::par(nv * nv, [&](int k) { L[k] = qm[k]; });
+  int* isc = E.si + M.L.i_iscal;
+  X::par(1, [&](int) { isc[1] = 0; });
+  if (!chol_block<X>(M, E, L, nv, 0, nv)) return;
+  Real* row = LD(tmp);
+  Real* x = LD(tmp2);
+  X::par(1, [&](int) {
+    auto quad = [&]() {
+      for (int i = 0; i < nv; i++) x[i] = row[i];
+      for (int i = 0; i < nv; i++) x[i] = (x[i] - sdot_n(L + i * nv, x, i)) / L[i * nv + i];
+      for (int i = nv - 1; i >= 0; i--) {
+        x[i] /= L[i * nv + i];
+        Real xi = x[i];
+        for (int k = 0; k < i; k++) x[k] += -xi * L[i * nv + k];
+      }
+      Real s = rsum0();
+      for (int i = 0; i < nv; i++) s += row[i] * x[i];
+      return s;
+    };
+    for (int k = 0; k < nv; k++) {
+      for (int i = 0; i < nv; i++) row[i] = 0.0;
+      row[k] = 1.0;
+      dof_iw[k] = quad();
+    }
+    for (int j = 0; j < M.njnt; j++) {
+      int da = M.jnt_dofadr[j], t = M.jnt_type[j];
+      int ng = t == J_FREE ? 2 : (t == J_BALL ? 1 : 0);
+      for (int gi = 0; gi < ng; gi++) {
+        int g = gi * 3;
+        Real s = rsum0();
+        for (int i = 0; i < 3; i++) s += dof_iw[da + g + i];
+        Real avg = s / 3.0;
+        for (int i = 0; i < 3; i++) dof_iw[da + g + i] = avg;
+      }
+    }
+    const Real* cd = LD(cdof);
+    for (int b = 1; b < nb; b++) {
+      Real w0 = 0.0, w1 = 0.0;
+      V3 xi = ld3(LD(xipos) + 3 * b);
+      for (int axis = 0; axis < 3; axis++) {
+        V3 dir = v3(axis == 0 ? 1.0 : 0.0, axis == 1 ? 1.0 : 0.0, axis == 2 ? 1.0 : 0.0);
+        for (int kind = 0; kind < 2; kind++) {
+          for (int i = 0; i < nv; i++) row[i] = 0.0;
+          int k = M.body_lastdof[b];
+          while (k >= 0) {
+            SP c = ldsp(cd + 6 * k);
+            row[k] = kind == 0 ? dot(dir, point_velocity(c, xi)) : dot(dir, c.a);
+            k = M.dof_parent[k];
+          }
+          Real qd = quad() / 3.0;
+          if (kind == 0) w0 += qd;
+          else w1 += qd;
+        }
+      }
+      body_iw[2 * b] = w0;
+      body_iw[2 * b + 1] = w1;
+    }
+  });
+}
+
+// ---------------- per-env driver (shared by the kernels and the serial policy) ----------------
+enum { IS_NCON = 0, IS_STATUS = 7, IS_ITERS = 8, IS_NROWS = 9 };
+
+template <class X>
+PHD void load_state(const DModel& M, const DState& S, Env& E) {
+  Real* s = E.s;
+  const int64_t e = E.e;
+  X::par(M.nq, [&](int k) { s[M.L.qpos + k] = S.qpos[e * M.nq + k]; });
+  X::par(M.nv, [&](int k) {
+    s[M.L.qvel + k] = S.qvel[e * M.nv + k];
+    s[M.L.qacc + k] = S.qacc[e * M.nv + k];
+    s[M.L.applied + k] = S.applied[e * M.nv + k];
+  });
+  X::par(M.nu, [&](int k) { s[M.L.ctrl + k] = S.ctrl[e * M.nu + k]; });
+  int* isc = E.si + M.L.i_iscal;
+  X::par(1, [&](int) {
+    for (int k = 0; k < 16; k++) isc[k] = 0;
+    isc[IS_NCON] = S.ncon[e];
+    isc[IS_STATUS] = S.status[e];
+    isc[IS_ITERS] = S.iters[e];
+    isc[IS_NROWS] = S.nrows[e];
+  });
+}
+
+template <class X>
+PHD void store_poses(const DModel& M, const DState& S, Env& E) {
+  const Real* s = E.s;
+  const int64_t e = E.e;
+  X::par(3 * M.ngeom, [&](int k) { S.geom_xpos[e * 3 * M.ngeom + k] = s[M.L.gxpos + k]; });
+  X::par(9 * M.ngeom, [&](int k) { S.geom_xmat[e * 9 * M.ngeom + k] = s[M.L.gxmat + k]; });
+  X::par(3 * M.nbody, [&](int k) { S.xpos[e * 3 * M.nbody + k] = s[M.L.xpos + k]; });
+  X::par(9 * M.nbody, [&](int k) { S.xmat[e * 9 * M.nbody + k] = s[M.L.xmat + k]; });
+}
+
+// Load one env, run nstep steps, refresh its poses and store it back.
+template <class X>
+PHD void run_env(const DModel& M, const DState& S, Env& E, int64_t nstep, const Real* ctrl_seq, Real* qpos_out,
+                 int64_t rel) {
+  Real* lds = E.s;
+  load_state<X>(M, S, E);
+  int* isc = E.si + M.L.i_iscal;
+  const int64_t e = E.e;
+  for (int64_t t = 0; t < nstep; t++) {
+    if (isc[IS_STATUS] != 0) break;  // a failed env stays put (the reference panics)
+    if (ctrl_seq) X::par(M.nu, [&](int k) { lds[M.L.ctrl + k] = ctrl_seq[(rel * nstep + t) * M.nu + k]; });
+    if (!step_env<X>(M, S, E)) break;
+    if (qpos_out) X::par(M.nq, [&](int k) { qpos_out[(rel * nstep + t) * M.nq + k] = lds[M.L.qpos + k]; });
+  }
+  kinematics<X>(M, E);
+  store_poses<X>(M, S, E);
+  X::par(M.nq, [&](int k) { S.qpos[e * M.nq + k] = lds[M.L.qpos + k]; });
+  X::par(M.nv, [&](int k) {
+    S.qvel[e * M.nv + k] = lds[M.L.qvel + k];
+    S.qacc[e * M.nv + k] = lds[M.L.qacc + k];
+  });
+  X::par(M.nu, [&](int k) { S.ctrl[e * M.nu + k] = lds[M.L.ctrl + k]; });
+  X::par(1, [&](int) {
+    S.ncon[e] = isc[IS_NCON];
+    S.status[e] = isc[IS_STATUS];
+    S.iters[e] = isc[IS_ITERS];
+    S.nrows[e] = isc[IS_NROWS];
+    S.time[e] += (Real)nstep * M.timestep;
+  });
+}
+
+#undef LD
+}  // namespace phk

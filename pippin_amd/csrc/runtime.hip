@@ -1,0 +1,923 @@
+// Device runtime and C ABI of the batched FP64 physics backend (include/pippin_hip.h).
+//
+// One environment per 64-lane wavefront (one workgroup); per-env working set in
+// LDS; per-env state, contacts and large row buffers in HBM. The C entry points
+// follow the `Physics` trait (crates/pippin-env/src/lib.rs:110-155) and the CPU
+// backend's conventions (crates/pippin-env/src/pippin_cpu.rs).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pippin_hip.h"
+#include "host_model.hpp"
+#include "kbody.hpp"
+
+using namespace phk;
+
+// ---------------------------------------------------------------- kernels
+struct DevX {
+  template <class F>
+  __device__ __forceinline__ static void par(int n, F&& f) {
+    for (int i = (int)threadIdx.x; i < n; i += 64) f(i);
+    __syncthreads();
+  }
+};
+
+// Advance envs [begin, begin + gridDim.x) by nstep steps. With ctrl_seq
+// (envs x nstep x nu) the controls change every step and qpos_out
+// (envs x nstep x nq) receives qpos after each step (Batch::rollout, batch.rs:46-59).
+__global__ void __launch_bounds__(64) k_step(DModel M, DState S, int64_t begin, int64_t nstep, const Real* ctrl_seq,
+                                             Real* qpos_out) {
+  extern __shared__ double lds[];
+  Env E;
+  E.s = lds;
+  E.si = reinterpret_cast<int*>(lds + M.L.ndouble);
+  E.e = begin + blockIdx.x;
+  E.var = M.env_var[E.e];
+  run_env<DevX>(M, S, E, nstep, ctrl_seq, qpos_out, blockIdx.x);
+}
+
+// Kinematics only (poses after set(qpos) / reset), for listed envs or a range.
+__global__ void __launch_bounds__(64) k_kin(DModel M, DState S, int64_t begin, const int64_t* ids) {
+  extern __shared__ double lds[];
+  Env E;
+  E.s = lds;
+  E.si = reinterpret_cast<int*>(lds + M.L.ndouble);
+  E.e = ids ? ids[blockIdx.x] : begin + blockIdx.x;
+  E.var = M.env_var[E.e];
+  load_state<DevX>(M, S, E);
+  kinematics<DevX>(M, E);
+  store_poses<DevX>(M, S, E);
+}
+
+// Data::reset (data.rs:122-131) for listed envs: qacc is kept.
+__global__ void __launch_bounds__(64) k_reset(DModel M, DState S, const int64_t* ids) {
+  const int64_t e = ids[blockIdx.x];
+  const int var = M.env_var[e];
+  for (int k = threadIdx.x; k < M.nq; k += 64) S.qpos[e * M.nq + k] = M.v_qpos_reset[(int64_t)var * M.nq + k];
+  for (int k = threadIdx.x; k < M.nv; k += 64) {
+    S.qvel[e * M.nv + k] = 0.0;
+    S.applied[e * M.nv + k] = 0.0;
+  }
+  for (int k = threadIdx.x; k < M.nu; k += 64) S.ctrl[e * M.nu + k] = 0.0;
+  if (threadIdx.x == 0) {
+    S.ncon[e] = 0;
+    S.npcon[e] = 0;
+    S.time[e] = 0.0;
+  }
+}
+
+// Forward dynamics up to RNE; copies M(q) and c(q, qdot) out.
+__global__ void __launch_bounds__(64) k_mass_bias(DModel M, DState S, int64_t begin, Real* qm, Real* bias) {
+  extern __shared__ double lds[];
+  Env E;
+  E.s = lds;
+  E.si = reinterpret_cast<int*>(lds + M.L.ndouble);
+  E.e = begin + blockIdx.x;
+  E.var = M.env_var[E.e];
+  load_state<DevX>(M, S, E);
+  kinematics<DevX>(M, E);
+  crba<DevX>(M, E);
+  velocity<DevX>(M, E);
+  rne<DevX>(M, E);
+  const int64_t rel = blockIdx.x;
+  DevX::par(M.nv * M.nv, [&](int k) { qm[rel * M.nv * M.nv + k] = lds[M.L.qm + k]; });
+  DevX::par(M.nv, [&](int k) { bias[rel * M.nv + k] = lds[M.L.bias + k]; });
+}
+
+// set_const for listed variants.
+__global__ void __launch_bounds__(64) k_setconst(DModel M, const int* vars, Real* dof_iw, Real* body_iw) {
+  extern __shared__ double lds[];
+  Env E;
+  E.s = lds;
+  E.si = reinterpret_cast<int*>(lds + M.L.ndouble);
+  E.e = 0;
+  E.var = vars[blockIdx.x];
+  DevX::par(M.nq, [&](int k) { lds[M.L.qpos + k] = M.v_qpos_reset[(int64_t)E.var * M.nq + k]; });
+  set_const_env<DevX>(M, E, dof_iw + (int64_t)E.var * M.nv, body_iw + (int64_t)E.var * M.nbody * 2);
+}
+
+// ---------------------------------------------------------------- host runtime
+static thread_local std::string g_err;
+void ph_set_error(const std::string& msg) { g_err = msg; }
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return fail(PH_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = count;
+    if (count == 0) count = 1;
+    return hipMalloc(&p, count * sizeof(T));
+  }
+  hipError_t upload(const std::vector<T>& v) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+  hipError_t zero() { return hipMemset(p, 0, (n ? n : 1) * sizeof(T)); }
+};
+
+struct Variants {
+  // host mirrors, [nvar][...]
+  std::vector<double> geom_size, geom_aabb, friction, body_mass, body_ipos, body_inertia, dof_damping, act_gain,
+      act_bias, dof_invweight, body_invweight, qpos_reset;
+  std::vector<uint32_t> contype, conaffinity;
+  std::vector<float> rgba;
+  std::vector<int> damped;
+  int nvar = 0;
+};
+
+struct ph_sim {
+  phh::HostModel hm;
+  int64_t n = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DModel M;
+  DState S;
+  Variants V;
+  std::vector<int> env_var;  // host mirror
+  std::vector<int> var_refs;  // envs per variant
+  // device buffers
+  DBuf<int> d_i[40];
+  DBuf<double> d_f[24];
+  DBuf<uint64_t> d_u64[2];
+  DBuf<uint32_t> d_u32[2];
+  DBuf<int> d_env_var;
+  // variant buffers
+  DBuf<double> dv_geom_size, dv_geom_aabb, dv_friction, dv_body_mass, dv_body_ipos, dv_body_inertia, dv_dof_damping,
+      dv_act_gain, dv_act_bias, dv_dof_invweight, dv_body_invweight, dv_qpos_reset;
+  DBuf<uint32_t> dv_contype, dv_conaffinity;
+  DBuf<int> dv_damped;
+  // state buffers
+  DBuf<double> s_qpos, s_qvel, s_ctrl, s_qacc, s_applied, s_time, s_gxpos, s_gxmat, s_xpos, s_xmat, s_con, s_pcon,
+      s_plambda, s_stage, s_gjac, s_grow;
+  DBuf<int> s_ncon, s_iters, s_nrows, s_status, s_cgeom, s_pcgeom, s_npcon;
+  DBuf<int64_t> d_ids;
+};
+
+extern "C" const char* pippin_hip_last_error(void) { return g_err.c_str(); }
+
+template <class T>
+static hipError_t up(DBuf<T>& b, const std::vector<T>& v, const T** out) {
+  hipError_t e = b.upload(v);
+  *out = b.p;
+  return e;
+}
+
+static int upload_variants(ph_sim* s) {
+  Variants& V = s->V;
+  DModel& M = s->M;
+  HIPCHK(up(s->dv_geom_size, V.geom_size, &M.v_geom_size));
+  HIPCHK(up(s->dv_geom_aabb, V.geom_aabb, &M.v_geom_aabb));
+  HIPCHK(up(s->dv_friction, V.friction, &M.v_friction));
+  HIPCHK(up(s->dv_contype, V.contype, &M.v_contype));
+  HIPCHK(up(s->dv_conaffinity, V.conaffinity, &M.v_conaffinity));
+  HIPCHK(up(s->dv_body_mass, V.body_mass, &M.v_body_mass));
+  HIPCHK(up(s->dv_body_ipos, V.body_ipos, &M.v_body_ipos));
+  HIPCHK(up(s->dv_body_inertia, V.body_inertia, &M.v_body_inertia));
+  HIPCHK(up(s->dv_dof_damping, V.dof_damping, &M.v_dof_damping));
+  HIPCHK(up(s->dv_act_gain, V.act_gain, &M.v_act_gain));
+  HIPCHK(up(s->dv_act_bias, V.act_bias, &M.v_act_bias));
+  HIPCHK(up(s->dv_dof_invweight, V.dof_invweight, &M.v_dof_invweight));
+  HIPCHK(up(s->dv_body_invweight, V.body_invweight, &M.v_body_invweight));
+  HIPCHK(up(s->dv_qpos_reset, V.qpos_reset, &M.v_qpos_reset));
+  HIPCHK(up(s->dv_damped, V.damped, &M.v_damped));
+  return PH_OK;
+}
+
+// Recompute invweight0 on the device for the listed variants and pull it back.
+static int run_setconst(ph_sim* s, const std::vector<int>& vars) {
+  if (vars.empty()) return PH_OK;
+  DBuf<int> dv;
+  HIPCHK(dv.upload(vars));
+  hipLaunchKernelGGL(k_setconst, dim3((unsigned)vars.size()), dim3(64), s->M.L.bytes, s->stream, s->M, dv.p,
+                     s->dv_dof_invweight.p, s->dv_body_invweight.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  Variants& V = s->V;
+  HIPCHK(hipMemcpy(V.dof_invweight.data(), s->dv_dof_invweight.p, V.dof_invweight.size() * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(V.body_invweight.data(), s->dv_body_invweight.p, V.body_invweight.size() * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  return PH_OK;
+}
+
+static void variant_init(ph_sim* s) {
+  const phh::HostModel& hm = s->hm;
+  Variants& V = s->V;
+  V.nvar = 1;
+  V.geom_size = hm.geom_size;
+  V.geom_aabb = hm.geom_aabb;
+  V.friction = hm.geom_friction;
+  V.contype = hm.geom_contype;
+  V.conaffinity = hm.geom_conaffinity;
+  V.rgba = hm.geom_rgba;
+  V.body_mass = hm.body_mass;
+  V.body_ipos = hm.body_ipos;
+  V.body_inertia = hm.body_inertia;
+  V.dof_damping = hm.dof_damping;
+  V.act_gain = hm.actuator_gain;
+  V.act_bias = hm.actuator_bias;
+  V.dof_invweight.assign(hm.d.nv, 0.0);
+  V.body_invweight.assign(2 * hm.d.nbody, 0.0);
+  V.qpos_reset = hm.qpos_reset;
+  V.damped.assign(1, hm.damped ? 1 : 0);
+}
+
+// Copy variant `src` to a new variant; returns its index.
+static int variant_clone(ph_sim* s, int src) {
+  Variants& V = s->V;
+  auto dup = [&](auto& vec, size_t w) {
+    vec.insert(vec.end(), vec.begin() + src * w, vec.begin() + (src + 1) * w);
+  };
+  const ph_model_desc& d = s->hm.d;
+  dup(V.geom_size, 3 * d.ngeom);
+  dup(V.geom_aabb, 6 * d.ngeom);
+  dup(V.friction, d.ngeom);
+  dup(V.contype, d.ngeom);
+  dup(V.conaffinity, d.ngeom);
+  dup(V.rgba, 4 * d.ngeom);
+  dup(V.body_mass, d.nbody);
+  dup(V.body_ipos, 3 * d.nbody);
+  dup(V.body_inertia, 9 * d.nbody);
+  dup(V.dof_damping, d.nv);
+  dup(V.act_gain, d.nu);
+  dup(V.act_bias, 3 * d.nu);
+  dup(V.dof_invweight, d.nv);
+  dup(V.body_invweight, 2 * d.nbody);
+  dup(V.qpos_reset, d.nq);
+  dup(V.damped, 1);
+  return V.nvar++;
+}
+
+extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, const ph_options* opt, ph_sim** out) {
+  if (!desc || !out || num_envs < 0) return fail(PH_ERR_ARG, "null argument or negative env count");
+  std::unique_ptr<ph_sim> s(new ph_sim());
+  std::string err;
+  if (!phh::build_host_model(*desc, s->hm, &err)) return fail(PH_ERR_MODEL, err);
+  phh::HostModel& hm = s->hm;
+  const ph_model_desc& d = hm.d;
+  int solver = d.solver_kind;
+  if (opt && opt->solver_override >= 0) solver = opt->solver_override;
+  if (solver != PH_SOLVER_NEWTON) return fail(PH_ERR_MODEL, "the device path implements the Newton solver only");
+  if (d.neq > 0 || hm.block_adr.size() > 2) {
+    // islands over several dof blocks (newton.rs:391-470) are not on the device yet
+    if (hm.block_adr.size() > 2) return fail(PH_ERR_MODEL, "models with several dof blocks are not supported yet");
+  }
+  s->n = num_envs;
+  s->device = opt ? opt->device : 0;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+
+  DModel& M = s->M;
+  phh::init_dmodel_scalars(hm, solver, opt ? opt->max_contacts : 0, M);
+  if (M.L.bytes > 64 * 1024) return fail(PH_ERR_MODEL, "model too large for the per-env LDS working set");
+
+  int ii = 0, fi = 0;
+  auto ui = [&](const std::vector<int32_t>& v, const int** dst) -> hipError_t { return up(s->d_i[ii++], v, dst); };
+  auto uf = [&](const std::vector<double>& v, const Real** dst) -> hipError_t { return up(s->d_f[fi++], v, dst); };
+  HIPCHK(ui(hm.body_parent, &M.body_parent));
+  HIPCHK(ui(hm.body_jntadr, &M.body_jntadr));
+  HIPCHK(ui(hm.body_jntnum, &M.body_jntnum));
+  HIPCHK(ui(hm.body_dofadr, &M.body_dofadr));
+  HIPCHK(ui(hm.body_dofnum, &M.body_dofnum));
+  HIPCHK(ui(hm.body_lastdof, &M.body_lastdof));
+  HIPCHK(ui(hm.level_adr, &M.level_adr));
+  HIPCHK(ui(hm.level_body, &M.level_body));
+  HIPCHK(ui(hm.jnt_type, &M.jnt_type));
+  HIPCHK(ui(hm.jnt_qposadr, &M.jnt_qposadr));
+  HIPCHK(ui(hm.jnt_dofadr, &M.jnt_dofadr));
+  HIPCHK(ui(hm.jnt_limited, &M.jnt_limited));
+  HIPCHK(ui(hm.dof_body, &M.dof_body));
+  HIPCHK(ui(hm.dof_parent, &M.dof_parent));
+  HIPCHK(ui(hm.dof_jnt, &M.dof_jnt));
+  HIPCHK(ui(hm.block_adr, &M.block_adr));
+  HIPCHK(ui(hm.geom_type, &M.geom_type));
+  HIPCHK(ui(hm.geom_body, &M.geom_body));
+  HIPCHK(ui(hm.pair_g1, &M.pair_g1));
+  HIPCHK(ui(hm.pair_g2, &M.pair_g2));
+  HIPCHK(ui(hm.actuator_ctrllimited, &M.act_ctrllimited));
+  HIPCHK(ui(hm.actuator_forcelimited, &M.act_forcelimited));
+  HIPCHK(ui(hm.actuator_moment_adr, &M.act_madr));
+  HIPCHK(ui(hm.actuator_moment_num, &M.act_mnum));
+  HIPCHK(ui(hm.moment_jnt, &M.mom_jnt));
+  HIPCHK(ui(hm.dofact_adr, &M.dofact_adr));
+  HIPCHK(ui(hm.dofact_num, &M.dofact_num));
+  HIPCHK(ui(hm.dofact_u, &M.dofact_u));
+  HIPCHK(ui(hm.eq_joint1, &M.eq_j1));
+  HIPCHK(ui(hm.eq_joint2, &M.eq_j2));
+  HIPCHK(uf(hm.body_pos, &M.body_pos));
+  HIPCHK(uf(hm.body_quat, &M.body_quat));
+  HIPCHK(uf(hm.jnt_pos, &M.jnt_pos));
+  HIPCHK(uf(hm.jnt_axis, &M.jnt_axis));
+  HIPCHK(uf(hm.jnt_range, &M.jnt_range));
+  HIPCHK(uf(hm.jnt_stiffness, &M.jnt_stiffness));
+  HIPCHK(uf(hm.jnt_springref, &M.jnt_springref));
+  HIPCHK(uf(hm.dof_armature, &M.dof_armature));
+  HIPCHK(uf(hm.geom_pos, &M.geom_pos));
+  HIPCHK(uf(hm.geom_quat, &M.geom_quat));
+  HIPCHK(uf(hm.actuator_gear, &M.act_gear));
+  HIPCHK(uf(hm.actuator_ctrlrange, &M.act_ctrlrange));
+  HIPCHK(uf(hm.actuator_forcerange, &M.act_forcerange));
+  HIPCHK(uf(hm.moment_coef, &M.mom_coef));
+  HIPCHK(uf(hm.dofact_coef, &M.dofact_coef));
+  HIPCHK(uf(hm.eq_polycoef, &M.eq_poly));
+  HIPCHK(uf(hm.eq_solref, &M.eq_solref));
+  HIPCHK(uf(hm.eq_solimp, &M.eq_solimp));
+  HIPCHK(uf(hm.qpos0, &M.qpos0));
+  HIPCHK(up(s->d_u64[0], hm.body_chain, &M.body_chain));
+  HIPCHK(up(s->d_u64[1], hm.dof_anc, &M.dof_anc));
+
+  variant_init(s.get());
+  s->env_var.assign(num_envs, 0);
+  s->var_refs.assign(1, (int)num_envs);
+  HIPCHK(up(s->d_env_var, s->env_var, &M.env_var));
+  if (int rc = upload_variants(s.get())) return rc;
+
+  // per-env state
+  const int64_t n = std::max<int64_t>(num_envs, 1);
+  DState& S = s->S;
+  S.n = num_envs;
+  std::vector<double> q0(n * d.nq);
+  for (int64_t e = 0; e < n; e++)
+    for (int k = 0; k < d.nq; k++) q0[e * d.nq + k] = hm.qpos_reset[k];
+  HIPCHK(s->s_qpos.upload(q0));
+  HIPCHK(s->s_qvel.alloc(n * d.nv));
+  HIPCHK(s->s_qvel.zero());
+  HIPCHK(s->s_ctrl.alloc(n * d.nu));
+  HIPCHK(s->s_ctrl.zero());
+  HIPCHK(s->s_qacc.alloc(n * d.nv));
+  HIPCHK(s->s_qacc.zero());
+  HIPCHK(s->s_applied.alloc(n * d.nv));
+  HIPCHK(s->s_applied.zero());
+  HIPCHK(s->s_time.alloc(n));
+  HIPCHK(s->s_time.zero());
+  HIPCHK(s->s_gxpos.alloc(n * 3 * d.ngeom));
+  HIPCHK(s->s_gxmat.alloc(n * 9 * d.ngeom));
+  HIPCHK(s->s_xpos.alloc(n * 3 * d.nbody));
+  HIPCHK(s->s_xmat.alloc(n * 9 * d.nbody));
+  HIPCHK(s->s_con.alloc(n * M.ncmax * CW));
+  HIPCHK(s->s_cgeom.alloc(n * M.ncmax * 3));
+  HIPCHK(s->s_pcon.alloc(n * M.ncmax * CW));
+  HIPCHK(s->s_pcgeom.alloc(n * M.ncmax * 3));
+  HIPCHK(s->s_plambda.alloc(n * M.ncmax * 3));
+  HIPCHK(s->s_stage.alloc(n * std::max(M.npair, 1) * MAXHIT * CW));
+  HIPCHK(s->s_gjac.alloc(n * (int64_t)M.nrmax * d.nv));
+  HIPCHK(s->s_grow.alloc(n * (int64_t)M.nrmax * 10));
+  for (auto* b : {&s->s_ncon, &s->s_iters, &s->s_nrows, &s->s_status, &s->s_npcon}) {
+    HIPCHK(b->alloc(n));
+    HIPCHK(b->zero());
+  }
+  S.qpos = s->s_qpos.p;
+  S.qvel = s->s_qvel.p;
+  S.ctrl = s->s_ctrl.p;
+  S.qacc = s->s_qacc.p;
+  S.applied = s->s_applied.p;
+  S.time = s->s_time.p;
+  S.geom_xpos = s->s_gxpos.p;
+  S.geom_xmat = s->s_gxmat.p;
+  S.xpos = s->s_xpos.p;
+  S.xmat = s->s_xmat.p;
+  S.ncon = s->s_ncon.p;
+  S.iters = s->s_iters.p;
+  S.nrows = s->s_nrows.p;
+  S.status = s->s_status.p;
+  S.con = s->s_con.p;
+  S.cgeom = s->s_cgeom.p;
+  S.pcon = s->s_pcon.p;
+  S.pcgeom = s->s_pcgeom.p;
+  S.plambda = s->s_plambda.p;
+  S.npcon = s->s_npcon.p;
+  S.stage = s->s_stage.p;
+  S.gjac = s->s_gjac.p;
+  S.grow = s->s_grow.p;
+
+  // invweight0 (forward::set_const) for the base variant, then poses (PippinCpu::new runs kinematics)
+  if (int rc = run_setconst(s.get(), {0})) return rc;
+  if (int rc = upload_variants(s.get())) return rc;
+  if (num_envs > 0) {
+    hipLaunchKernelGGL(k_kin, dim3((unsigned)num_envs), dim3(64), M.L.bytes, s->stream, M, S, (int64_t)0,
+                       (const int64_t*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  *out = s.release();
+  return PH_OK;
+}
+
+extern "C" void pippin_hip_destroy(ph_sim* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->stream);
+  hipStream_t st = s->stream;
+  delete s;
+  if (st) (void)hipStreamDestroy(st);
+}
+
+extern "C" int64_t pippin_hip_num_envs(const ph_sim* s) { return s ? s->n : -1; }
+
+extern "C" int pippin_hip_dims(const ph_sim* s, ph_dims* out) {
+  if (!s || !out) return fail(PH_ERR_ARG, "null argument");
+  out->nq = s->hm.d.nq;
+  out->nv = s->hm.d.nv;
+  out->nu = s->hm.d.nu;
+  out->ngeom = s->hm.d.ngeom;
+  out->ncam = s->hm.d.ncam;
+  return PH_OK;
+}
+
+static int check_range(const ph_sim* s, int64_t b, int64_t e) {
+  if (!s) return fail(PH_ERR_ARG, "null sim");
+  if (b < 0 || e < b || e > s->n) return fail(PH_ERR_ARG, "env range out of bounds");
+  return PH_OK;
+}
+static int check_len(const char* what, int64_t got, int64_t envs, int64_t width) {
+  if (got != envs * width)
+    return fail(PH_ERR_SIZE, std::string(what) + ": buffer has " + std::to_string(got) + " elements, expected " +
+                                 std::to_string(envs) + " x " + std::to_string(width));
+  return PH_OK;
+}
+
+static int status_of(ph_sim* s, int64_t b, int64_t e) {
+  std::vector<int> st(e - b);
+  if (e > b) HIPCHK(hipMemcpy(st.data(), s->s_status.p + b, (e - b) * sizeof(int), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < e - b; i++) {
+    if (st[i] & ST_NOTPD) return fail(PH_ERR_NOTPD, "mass matrix is not positive definite (env " + std::to_string(b + i) + ")");
+    if (st[i] & (ST_ROW_OVERFLOW | ST_CONTACT_OVERFLOW))
+      return fail(PH_ERR_DEVICE, "constraint capacity exceeded (env " + std::to_string(b + i) + ")");
+  }
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  if (nstep < 0) return fail(PH_ERR_ARG, "negative step count");
+  if (end == begin) return PH_OK;
+  HIPCHK(hipSetDevice(s->device));
+  hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin,
+                     nstep, (const Real*)nullptr, (Real*)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return status_of(s, begin, end);
+}
+
+extern "C" int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const double* ctrl,
+                                         double* qpos_out) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  if (nstep < 0 || !ctrl) return fail(PH_ERR_ARG, "bad rollout arguments");
+  if (end == begin) return PH_OK;
+  HIPCHK(hipSetDevice(s->device));
+  hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin,
+                     nstep, ctrl, qpos_out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return status_of(s, begin, end);
+}
+
+extern "C" int pippin_hip_state_device(ph_sim* s, int32_t field, double** ptr) {
+  if (!s || !ptr) return fail(PH_ERR_ARG, "null argument");
+  *ptr = field == PH_FIELD_QPOS ? s->s_qpos.p : field == PH_FIELD_QVEL ? s->s_qvel.p : field == PH_FIELD_CTRL ? s->s_ctrl.p : nullptr;
+  return *ptr ? PH_OK : fail(PH_ERR_ARG, "unknown field");
+}
+
+extern "C" void* pippin_hip_stream(ph_sim* s) { return s ? (void*)s->stream : nullptr; }
+
+static int field_width(const ph_sim* s, int32_t f, double** dptr) {
+  switch (f) {
+    case PH_FIELD_QPOS: *dptr = s->s_qpos.p; return s->hm.d.nq;
+    case PH_FIELD_QVEL: *dptr = s->s_qvel.p; return s->hm.d.nv;
+    case PH_FIELD_CTRL: *dptr = s->s_ctrl.p; return s->hm.d.nu;
+    default: return -1;
+  }
+}
+
+extern "C" int pippin_hip_get(const ph_sim* s, int32_t field, int64_t begin, int64_t end, double* out, int64_t len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  double* d;
+  int w = field_width(s, field, &d);
+  if (w < 0) return fail(PH_ERR_ARG, "unknown field");
+  if (int rc = check_len("get", len, end - begin, w)) return rc;
+  if (w == 0 || end == begin) return PH_OK;
+  HIPCHK(hipMemcpy(out, d + begin * w, (end - begin) * w * sizeof(double), hipMemcpyDeviceToHost));
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_set(ph_sim* s, int32_t field, int64_t begin, int64_t end, const double* src, int64_t len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  double* d;
+  int w = field_width(s, field, &d);
+  if (w < 0) return fail(PH_ERR_ARG, "unknown field");
+  if (int rc = check_len("set", len, end - begin, w)) return rc;
+  if (w == 0 || end == begin) return PH_OK;
+  HIPCHK(hipMemcpy(d + begin * w, src, (end - begin) * w * sizeof(double), hipMemcpyHostToDevice));
+  if (field == PH_FIELD_QPOS) {
+    hipLaunchKernelGGL(k_kin, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin,
+                       (const int64_t*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_reset(ph_sim* s, const int64_t* envs, int64_t n) {
+  if (!s || (n > 0 && !envs)) return fail(PH_ERR_ARG, "null argument");
+  if (n <= 0) return PH_OK;
+  for (int64_t i = 0; i < n; i++)
+    if (envs[i] < 0 || envs[i] >= s->n) return fail(PH_ERR_ARG, "env id out of range");
+  std::vector<int64_t> ids(envs, envs + n);
+  HIPCHK(s->d_ids.upload(ids));
+  hipLaunchKernelGGL(k_reset, dim3((unsigned)n), dim3(64), 0, s->stream, s->M, s->S, s->d_ids.p);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_kin, dim3((unsigned)n), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, (int64_t)0, s->d_ids.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_poses(const ph_sim* s, int64_t begin, int64_t end, ph_pose* geoms, int64_t ngl, ph_pose* cams,
+                                int64_t ncl) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  const ph_model_desc& d = s->hm.d;
+  if (int rc = check_len("poses(geoms)", ngl, end - begin, d.ngeom)) return rc;
+  if (int rc = check_len("poses(cams)", ncl, end - begin, d.ncam)) return rc;
+  int64_t n = end - begin;
+  if (n == 0) return PH_OK;
+  std::vector<double> gp(n * 3 * d.ngeom), gm(n * 9 * d.ngeom), bp(n * 3 * d.nbody), bm(n * 9 * d.nbody);
+  if (d.ngeom) {
+    HIPCHK(hipMemcpy(gp.data(), s->s_gxpos.p + begin * 3 * d.ngeom, gp.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gm.data(), s->s_gxmat.p + begin * 9 * d.ngeom, gm.size() * 8, hipMemcpyDeviceToHost));
+  }
+  if (d.ncam) {
+    HIPCHK(hipMemcpy(bp.data(), s->s_xpos.p + begin * 3 * d.nbody, bp.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bm.data(), s->s_xmat.p + begin * 9 * d.nbody, bm.size() * 8, hipMemcpyDeviceToHost));
+  }
+  for (int64_t i = 0; i < n; i++) {
+    for (int g = 0; g < d.ngeom; g++) {
+      ph_pose& p = geoms[i * d.ngeom + g];
+      for (int k = 0; k < 3; k++) p.pos[k] = (float)gp[(i * d.ngeom + g) * 3 + k];
+      for (int k = 0; k < 9; k++) p.mat[k] = (float)gm[(i * d.ngeom + g) * 9 + k];
+    }
+    // camera pose from its body (pippin_cpu.rs:441-446)
+    for (int c = 0; c < d.ncam; c++) {
+      int b = s->hm.cam_body[c];
+      M3 xm = ldm(&bm[(i * d.nbody + b) * 9]);
+      V3 pos = ld3(&bp[(i * d.nbody + b) * 3]) + mul_vec(xm, ld3(&s->hm.cam_pos[3 * c]));
+      M3 mat = mul_mat(xm, to_mat(ldq(&s->hm.cam_quat[4 * c])));
+      ph_pose& p = cams[i * d.ncam + c];
+      p.pos[0] = (float)pos.x;
+      p.pos[1] = (float)pos.y;
+      p.pos[2] = (float)pos.z;
+      for (int k = 0; k < 9; k++) p.mat[k] = (float)mat.m[k];
+    }
+  }
+  return PH_OK;
+}
+
+// ---------------------------------------------------------------- parameters (params.rs)
+static int param_width(const ph_sim* s, int p) {
+  switch (p) {
+    case PH_PARAM_GEOM_SIZE:
+    case PH_PARAM_ACTUATOR_BIAS:
+    case PH_PARAM_BODY_INERTIA:
+    case PH_PARAM_BODY_IPOS: return 3;
+    case PH_PARAM_GEOM_RGBA: return 4;
+    case PH_PARAM_QPOS0: return s->hm.d.nq;
+    default: return 1;
+  }
+}
+static int64_t param_count(const ph_sim* s, int p) {
+  const ph_model_desc& d = s->hm.d;
+  switch (p) {
+    case PH_PARAM_GEOM_FRICTION:
+    case PH_PARAM_GEOM_SIZE:
+    case PH_PARAM_GEOM_RGBA:
+    case PH_PARAM_GEOM_CONTYPE:
+    case PH_PARAM_GEOM_CONAFFINITY: return d.ngeom;
+    case PH_PARAM_BODY_MASS:
+    case PH_PARAM_BODY_INERTIA:
+    case PH_PARAM_BODY_IPOS: return d.nbody;
+    case PH_PARAM_QPOS0: return 1;
+    case PH_PARAM_DOF_DAMPING: return d.nv;
+    case PH_PARAM_ACTUATOR_GAIN:
+    case PH_PARAM_ACTUATOR_BIAS: return d.nu;
+    default: return -1;
+  }
+}
+static const char* param_name(int p) {
+  static const char* names[] = {"GeomFriction", "GeomSize", "GeomRgba", "GeomContype", "GeomConaffinity", "BodyMass",
+                                "DofDamping", "ActuatorGain", "ActuatorBias", "BodyInertia", "BodyIpos", "Qpos0"};
+  return (p >= 0 && p < 12) ? names[p] : "?";
+}
+static uint32_t sat_u32(double x) {
+  if (!(x > 0.0)) return 0;
+  if (x >= 4294967295.0) return 4294967295u;
+  return (uint32_t)x;
+}
+
+// Validate one element edit against variant `v` (params.rs:111-177) and,
+// when `apply`, perform it. Returns PH_OK or PH_ERR_ARG with the message.
+static int param_apply(ph_sim* s, int v, int p, int64_t id, const double* x, bool apply, bool* need_setconst) {
+  Variants& V = s->V;
+  const ph_model_desc& d = s->hm.d;
+  int w = param_width(s, p);
+  for (int i = 0; i < w; i++)
+    if (!std::isfinite(x[i])) return fail(PH_ERR_ARG, std::string(param_name(p)) + ": values must be finite");
+  switch (p) {
+    case PH_PARAM_GEOM_FRICTION:
+      if (apply) V.friction[(int64_t)v * d.ngeom + id] = std::fmax(x[0], 0.0);
+      break;
+    case PH_PARAM_GEOM_SIZE: {
+      int t = s->hm.geom_type[id];
+      if (t == PH_GEOM_MESH || t == PH_GEOM_PLANE)
+        return fail(PH_ERR_ARG, "geom_size applies to sphere, capsule, box, and cylinder geoms");
+      for (int i = 0; i < 3; i++)
+        if (x[i] < 0.0) return fail(PH_ERR_ARG, "geom_size must be non-negative");
+      if (apply) {
+        double* sz = &V.geom_size[((int64_t)v * d.ngeom + id) * 3];
+        for (int i = 0; i < 3; i++) sz[i] = x[i];
+        phh::local_aabb(t, sz, &V.geom_aabb[((int64_t)v * d.ngeom + id) * 6]);
+      }
+      break;
+    }
+    case PH_PARAM_GEOM_RGBA:
+      if (apply)
+        for (int i = 0; i < 4; i++) V.rgba[((int64_t)v * d.ngeom + id) * 4 + i] = (float)x[i];
+      break;
+    case PH_PARAM_GEOM_CONTYPE:
+    case PH_PARAM_GEOM_CONAFFINITY:
+      if (apply) {
+        uint32_t bits = sat_u32(std::fmax(x[0], 0.0));
+        (p == PH_PARAM_GEOM_CONTYPE ? V.contype : V.conaffinity)[(int64_t)v * d.ngeom + id] = bits;
+      }
+      break;
+    case PH_PARAM_BODY_MASS: {
+      if (id == 0) return fail(PH_ERR_ARG, "the world body has no mass");
+      if (x[0] <= 0.0 && s->hm.body_jntnum[id] > 0) return fail(PH_ERR_ARG, "moving bodies need positive mass");
+      if (apply) {
+        double& m = V.body_mass[(int64_t)v * d.nbody + id];
+        double scale = m > 0.0 ? x[0] / m : 1.0;
+        m = x[0];
+        double* I = &V.body_inertia[((int64_t)v * d.nbody + id) * 9];
+        for (int i = 0; i < 9; i++) I[i] = I[i] * scale;
+        *need_setconst = true;
+      }
+      break;
+    }
+    case PH_PARAM_DOF_DAMPING:
+      if (apply) V.dof_damping[(int64_t)v * d.nv + id] = std::fmax(x[0], 0.0);
+      break;
+    case PH_PARAM_ACTUATOR_GAIN:
+      if (apply) V.act_gain[(int64_t)v * d.nu + id] = x[0];
+      break;
+    case PH_PARAM_ACTUATOR_BIAS:
+      if (apply)
+        for (int i = 0; i < 3; i++) V.act_bias[((int64_t)v * d.nu + id) * 3 + i] = x[i];
+      break;
+    case PH_PARAM_BODY_INERTIA:
+      if (id == 0 || x[0] < 0.0 || x[1] < 0.0 || x[2] < 0.0)
+        return fail(PH_ERR_ARG, "body_inertia needs a non-world body and non-negative moments");
+      if (apply) {
+        double* I = &V.body_inertia[((int64_t)v * d.nbody + id) * 9];
+        for (int i = 0; i < 9; i++) I[i] = 0.0;
+        I[0] = x[0];
+        I[4] = x[1];
+        I[8] = x[2];
+        *need_setconst = true;
+      }
+      break;
+    case PH_PARAM_BODY_IPOS:
+      if (id == 0) return fail(PH_ERR_ARG, "the world body has no center of mass");
+      if (apply) {
+        for (int i = 0; i < 3; i++) V.body_ipos[((int64_t)v * d.nbody + id) * 3 + i] = x[i];
+        *need_setconst = true;
+      }
+      break;
+    case PH_PARAM_QPOS0:
+      if (apply)
+        for (int i = 0; i < d.nq; i++) V.qpos_reset[(int64_t)v * d.nq + i] = x[i];
+      break;
+    default: return fail(PH_ERR_ARG, "unknown parameter");
+  }
+  if (apply) {
+    V.damped[v] = phh::has_implicit_terms(s->hm, &V.dof_damping[(int64_t)v * d.nv], &V.act_bias[(int64_t)v * d.nu * 3])
+                      ? 1 : 0;
+  }
+  return PH_OK;
+}
+
+// pippin_cpu.rs:365-374: validate on the base values first (no partial update), then copy on write per env.
+extern "C" int pippin_hip_set_param(ph_sim* s, int32_t p, int64_t id, int64_t begin, int64_t end,
+                                    const double* values, int64_t len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  int64_t cnt = param_count(s, p);
+  if (cnt < 0) return fail(PH_ERR_ARG, "unknown parameter");
+  int w = param_width(s, p);
+  if (int rc = check_len("set_param", len, end - begin, w)) return rc;
+  if (id < 0 || id >= cnt)
+    return fail(PH_ERR_ARG, std::string(param_name(p)) + ": id " + std::to_string(id) + " out of range (have " +
+                                std::to_string(cnt) + ")");
+  bool dummy = false;
+  if (end > begin)
+    if (int rc = param_apply(s, 0, p, id, values, false, &dummy)) return rc;
+  for (int64_t i = 0; i < end - begin; i++)
+    if (int rc = param_apply(s, s->env_var[begin + i], p, id, values + i * w, false, &dummy)) return rc;
+  std::vector<int> recompute;
+  for (int64_t i = 0; i < end - begin; i++) {
+    int64_t e = begin + i;
+    int v = s->env_var[e];
+    if (s->var_refs[v] > 1) {
+      int nv = variant_clone(s, v);
+      s->var_refs[v]--;
+      s->var_refs.push_back(1);
+      s->env_var[e] = nv;
+      v = nv;
+    }
+    bool sc = false;
+    param_apply(s, v, p, id, values + i * w, true, &sc);
+    if (sc) recompute.push_back(v);
+  }
+  HIPCHK(up(s->d_env_var, s->env_var, &s->M.env_var));
+  if (int rc = upload_variants(s)) return rc;
+  if (!recompute.empty()) {
+    if (int rc = run_setconst(s, recompute)) return rc;
+    if (int rc = upload_variants(s)) return rc;
+  }
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_get_param(const ph_sim* s, int32_t p, int64_t id, int64_t begin, int64_t end, double* out,
+                                    int64_t len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  int64_t cnt = param_count(s, p);
+  if (cnt < 0) return fail(PH_ERR_ARG, "unknown parameter");
+  int w = param_width(s, p);
+  if (int rc = check_len("get_param", len, end - begin, w)) return rc;
+  if (id < 0 || id >= cnt) return fail(PH_ERR_ARG, std::string(param_name(p)) + ": id out of range");
+  const Variants& V = s->V;
+  const ph_model_desc& d = s->hm.d;
+  for (int64_t i = 0; i < end - begin; i++) {
+    int64_t v = s->env_var[begin + i];
+    double* o = out + i * w;
+    switch (p) {
+      case PH_PARAM_GEOM_FRICTION: o[0] = V.friction[v * d.ngeom + id]; break;
+      case PH_PARAM_GEOM_SIZE:
+        for (int k = 0; k < 3; k++) o[k] = V.geom_size[(v * d.ngeom + id) * 3 + k];
+        break;
+      case PH_PARAM_GEOM_RGBA:
+        for (int k = 0; k < 4; k++) o[k] = V.rgba[(v * d.ngeom + id) * 4 + k];
+        break;
+      case PH_PARAM_GEOM_CONTYPE: o[0] = V.contype[v * d.ngeom + id]; break;
+      case PH_PARAM_GEOM_CONAFFINITY: o[0] = V.conaffinity[v * d.ngeom + id]; break;
+      case PH_PARAM_BODY_MASS: o[0] = V.body_mass[v * d.nbody + id]; break;
+      case PH_PARAM_DOF_DAMPING: o[0] = V.dof_damping[v * d.nv + id]; break;
+      case PH_PARAM_ACTUATOR_GAIN: o[0] = V.act_gain[v * d.nu + id]; break;
+      case PH_PARAM_ACTUATOR_BIAS:
+        for (int k = 0; k < 3; k++) o[k] = V.act_bias[(v * d.nu + id) * 3 + k];
+        break;
+      case PH_PARAM_BODY_INERTIA:
+        for (int k = 0; k < 3; k++) o[k] = V.body_inertia[(v * d.nbody + id) * 9 + 4 * k];
+        break;
+      case PH_PARAM_BODY_IPOS:
+        for (int k = 0; k < 3; k++) o[k] = V.body_ipos[(v * d.nbody + id) * 3 + k];
+        break;
+      case PH_PARAM_QPOS0:
+        for (int k = 0; k < d.nq; k++) o[k] = V.qpos_reset[v * d.nq + k];
+        break;
+    }
+  }
+  return PH_OK;
+}
+
+// pippin_cpu.rs:388-400
+extern "C" int pippin_hip_appearance(const ph_sim* s, int64_t begin, int64_t end, ph_appearance* out, int64_t len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  const ph_model_desc& d = s->hm.d;
+  if (int rc = check_len("appearance", len, end - begin, d.ngeom)) return rc;
+  for (int64_t i = 0; i < end - begin; i++) {
+    int64_t v = s->env_var[begin + i];
+    for (int g = 0; g < d.ngeom; g++) {
+      ph_appearance& a = out[i * d.ngeom + g];
+      for (int k = 0; k < 4; k++) a.rgba[k] = s->V.rgba[(v * d.ngeom + g) * 4 + k];
+      for (int k = 0; k < 3; k++) {
+        double base = s->hm.geom_size[3 * g + k];
+        a.scale[k] = base > 0.0 ? (float)(s->V.geom_size[(v * d.ngeom + g) * 3 + k] / base) : 1.0f;
+      }
+      a.pad = 0.0f;
+    }
+  }
+  return PH_OK;
+}
+
+// ---------------------------------------------------------------- diagnostics
+extern "C" int pippin_hip_ncon(const ph_sim* s, int64_t begin, int64_t end, int32_t* out, int64_t len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  if (int rc = check_len("ncon", len, end - begin, 1)) return rc;
+  if (end > begin) HIPCHK(hipMemcpy(out, s->s_ncon.p + begin, (end - begin) * sizeof(int), hipMemcpyDeviceToHost));
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_solver_iterations(const ph_sim* s, int64_t begin, int64_t end, int32_t* out, int64_t len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  if (int rc = check_len("solver_iterations", len, end - begin, 1)) return rc;
+  if (end > begin) HIPCHK(hipMemcpy(out, s->s_iters.p + begin, (end - begin) * sizeof(int), hipMemcpyDeviceToHost));
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_contacts(const ph_sim* s, int64_t env, ph_contact* out, int32_t cap, int32_t* n) {
+  if (int rc = check_range(s, env, env + 1)) return rc;
+  int nc = 0;
+  HIPCHK(hipMemcpy(&nc, s->s_ncon.p + env, sizeof(int), hipMemcpyDeviceToHost));
+  *n = nc;
+  int m = std::min(nc, (int)cap);
+  if (m <= 0) return PH_OK;
+  std::vector<double> c(m * CW);
+  std::vector<int> g(m * 3);
+  HIPCHK(hipMemcpy(c.data(), s->s_con.p + env * s->M.ncmax * CW, c.size() * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(g.data(), s->s_cgeom.p + env * s->M.ncmax * 3, g.size() * 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < m; i++) {
+    for (int k = 0; k < 3; k++) {
+      out[i].pos[k] = c[i * CW + k];
+      out[i].normal[k] = c[i * CW + 3 + k];
+      out[i].tangent[k] = c[i * CW + 6 + k];
+    }
+    out[i].depth = c[i * CW + 9];
+    out[i].geom1 = g[3 * i];
+    out[i].geom2 = g[3 * i + 1];
+    out[i].feature = g[3 * i + 2];
+    out[i].pad = 0;
+  }
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_mass_bias(ph_sim* s, int64_t begin, int64_t end, double* qm, int64_t qm_len, double* bias,
+                                    int64_t bias_len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  int nv = s->hm.d.nv;
+  if (int rc = check_len("mass_bias(qm)", qm_len, end - begin, (int64_t)nv * nv)) return rc;
+  if (int rc = check_len("mass_bias(bias)", bias_len, end - begin, nv)) return rc;
+  int64_t n = end - begin;
+  if (n == 0) return PH_OK;
+  DBuf<double> dq, db;
+  HIPCHK(dq.alloc(n * nv * nv));
+  HIPCHK(db.alloc(n * nv));
+  hipLaunchKernelGGL(k_mass_bias, dim3((unsigned)n), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin, dq.p, db.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(qm, dq.p, n * nv * nv * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(bias, db.p, n * nv * 8, hipMemcpyDeviceToHost));
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_invweight(const ph_sim* s, int64_t begin, int64_t end, double* dof, int64_t dof_len,
+                                    double* body, int64_t body_len) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  const ph_model_desc& d = s->hm.d;
+  if (int rc = check_len("invweight(dof)", dof_len, end - begin, d.nv)) return rc;
+  if (int rc = check_len("invweight(body)", body_len, end - begin, 2 * d.nbody)) return rc;
+  for (int64_t i = 0; i < end - begin; i++) {
+    int64_t v = s->env_var[begin + i];
+    for (int k = 0; k < d.nv; k++) dof[i * d.nv + k] = s->V.dof_invweight[v * d.nv + k];
+    for (int k = 0; k < 2 * d.nbody; k++) body[i * 2 * d.nbody + k] = s->V.body_invweight[v * 2 * d.nbody + k];
+  }
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_collision_pairs(const ph_sim* s, int64_t env, int32_t* pairs, int32_t cap, int32_t* n) {
+  if (int rc = check_range(s, env, env + 1)) return rc;
+  int64_t v = s->env_var[env];
+  const ph_model_desc& d = s->hm.d;
+  int cnt = 0;
+  for (size_t p = 0; p < s->hm.pair_g1.size(); p++) {
+    int g1 = s->hm.pair_g1[p], g2 = s->hm.pair_g2[p];
+    uint32_t t1 = s->V.contype[v * d.ngeom + g1], a1 = s->V.conaffinity[v * d.ngeom + g1];
+    uint32_t t2 = s->V.contype[v * d.ngeom + g2], a2 = s->V.conaffinity[v * d.ngeom + g2];
+    if ((t1 & a2) == 0 && (t2 & a1) == 0) continue;
+    if (cnt < cap) {
+      pairs[2 * cnt] = g1;
+      pairs[2 * cnt + 1] = g2;
+    }
+    cnt++;
+  }
+  *n = cnt;
+  return PH_OK;
+}

@@ -1,0 +1,265 @@
+"""Python front end of libpippin_hip.so, mirroring Pippin's `Physics` trait.
+
+`HipPhysics` exposes the methods of `trait Physics` (crates/pippin-env/src/lib.rs:110-155)
+with PippinCpu's conventions (crates/pippin-env/src/pippin_cpu.rs): buffers shaped
+(envs, width), size mismatches raise `ValueError` (the trait panics through
+`check_len`), parameter errors raise `ParamError` (the trait returns `Err(String)`).
+All stepping runs in the HIP kernels; there is no CPU path behind this class, and
+constructing it fails loudly when the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .desc import (APPEARANCE_DTYPE, CONTACT_DTYPE, POSE_DTYPE, Desc, arrays_from_desc, ph_dims,
+                   ph_model_desc, ph_options)
+
+LIB_PATH = Path(__file__).resolve().parent / "libpippin_hip.so"
+FIELD = {"qpos": 0, "qvel": 1, "ctrl": 2}
+PARAMS = ["geom_friction", "geom_size", "geom_rgba", "geom_contype", "geom_conaffinity", "body_mass",
+          "dof_damping", "actuator_gain", "actuator_bias", "body_inertia", "body_ipos", "qpos0"]
+PARAM = {n: i for i, n in enumerate(PARAMS)}
+SOLVER = {"newton": 0, "pgs": 1}
+
+PH_ERR_SIZE = -2
+
+_lib = None
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class ParamError(ValueError):
+    pass
+
+
+def lib():
+    """Load libpippin_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise HipError(f"{LIB_PATH} is missing: build it with `python -m pippin_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
+        sigs = {
+            "pippin_hip_model_load_xml": ([C.c_char_p, C.c_char_p, C.POINTER(vp)], C.c_int),
+            "pippin_hip_model_load_file": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+            "pippin_hip_model_desc": ([vp], C.POINTER(ph_model_desc)),
+            "pippin_hip_model_free": ([vp], None),
+            "pippin_hip_create": ([vp, i64, vp, C.POINTER(vp)], C.c_int),
+            "pippin_hip_destroy": ([vp], None),
+            "pippin_hip_dims": ([vp, vp], C.c_int),
+            "pippin_hip_num_envs": ([vp], i64),
+            "pippin_hip_step": ([vp, i64, i64, i64], C.c_int),
+            "pippin_hip_reset": ([vp, vp, i64], C.c_int),
+            "pippin_hip_get": ([vp, i32, i64, i64, vp, i64], C.c_int),
+            "pippin_hip_set": ([vp, i32, i64, i64, vp, i64], C.c_int),
+            "pippin_hip_poses": ([vp, i64, i64, vp, i64, vp, i64], C.c_int),
+            "pippin_hip_set_param": ([vp, i32, i64, i64, i64, vp, i64], C.c_int),
+            "pippin_hip_get_param": ([vp, i32, i64, i64, i64, vp, i64], C.c_int),
+            "pippin_hip_appearance": ([vp, i64, i64, vp, i64], C.c_int),
+            "pippin_hip_ncon": ([vp, i64, i64, vp, i64], C.c_int),
+            "pippin_hip_contacts": ([vp, i64, vp, i32, vp], C.c_int),
+            "pippin_hip_solver_iterations": ([vp, i64, i64, vp, i64], C.c_int),
+            "pippin_hip_mass_bias": ([vp, i64, i64, vp, i64, vp, i64], C.c_int),
+            "pippin_hip_invweight": ([vp, i64, i64, vp, i64, vp, i64], C.c_int),
+            "pippin_hip_collision_pairs": ([vp, i64, vp, i32, vp], C.c_int),
+            "pippin_hip_rollout_device": ([vp, i64, i64, i64, vp, vp], C.c_int),
+            "pippin_hip_state_device": ([vp, i32, C.POINTER(vp)], C.c_int),
+            "pippin_hip_stream": ([vp], vp),
+            "pippin_hip_last_error": ([], C.c_char_p),
+        }
+        for name, (args, res) in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _err():
+    return lib().pippin_hip_last_error().decode()
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def load_model(path_or_xml: str | Path, is_xml: bool = False) -> dict:
+    """Parse MJCF with the library's loader; returns the raw descriptor as numpy arrays."""
+    L = lib()
+    h = C.c_void_p()
+    if is_xml:
+        rc = L.pippin_hip_model_load_xml(str(path_or_xml).encode(), b".", C.byref(h))
+    else:
+        rc = L.pippin_hip_model_load_file(str(path_or_xml).encode(), C.byref(h))
+    if rc != 0:
+        raise HipError(_err())
+    try:
+        return arrays_from_desc(L.pippin_hip_model_desc(h).contents)
+    finally:
+        L.pippin_hip_model_free(h)
+
+
+class HipPhysics:
+    """A batch of environments stepped by the HIP kernels on one GPU."""
+
+    name = "pippin-hip"
+
+    def __init__(self, model, num_envs: int, device: int = 0, solver: str | None = None,
+                 max_contacts: int = 0):
+        if isinstance(model, (str, Path)):
+            model = load_model(model)
+        self.arrays = model
+        self._desc = Desc(model)
+        opt = ph_options()
+        opt.device = device
+        opt.solver_override = -1 if solver is None else SOLVER[solver]
+        opt.max_contacts = max_contacts
+        self.L = lib()
+        self.h = C.c_void_p()
+        rc = self.L.pippin_hip_create(self._desc.ptr(), int(num_envs), C.byref(opt), C.byref(self.h))
+        if rc != 0:
+            self.h = None
+            raise HipError(_err())
+        d = ph_dims()
+        self._check(self.L.pippin_hip_dims(self.h, C.byref(d)))
+        self.nq, self.nv, self.nu, self.ngeom, self.ncam = d.nq, d.nv, d.nu, d.ngeom, d.ncam
+        self.nbody = int(model["nbody"])
+        self.n = int(num_envs)
+
+    @classmethod
+    def from_xml(cls, xml: str, num_envs: int, **kw):
+        return cls(load_model(xml, is_xml=True), num_envs, **kw)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pippin_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc == PH_ERR_SIZE:
+            raise ValueError(_err())
+        if rc != 0:
+            raise HipError(_err())
+
+    def _range(self, begin, end):
+        return (0, self.n) if begin is None else (begin, self.n if end is None else end)
+
+    # ---- Physics trait ----
+    def num_envs(self):
+        return self.n
+
+    def dims(self):
+        return dict(nq=self.nq, nv=self.nv, nu=self.nu, ngeom=self.ngeom, ncam=self.ncam)
+
+    def width(self, field):
+        return {"qpos": self.nq, "qvel": self.nv, "ctrl": self.nu}[field]
+
+    def step(self, begin, end, nstep=1):
+        self._check(self.L.pippin_hip_step(self.h, begin, end, nstep))
+
+    def reset(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        self._check(self.L.pippin_hip_reset(self.h, _p(ids), len(ids)))
+
+    def get(self, field, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        out = np.zeros((end - begin, self.width(field)))
+        self._check(self.L.pippin_hip_get(self.h, FIELD[field], begin, end, _p(out), out.size))
+        return out
+
+    def set(self, field, begin, end, src):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        self._check(self.L.pippin_hip_set(self.h, FIELD[field], begin, end, _p(src), src.size))
+
+    def poses(self, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        g = np.zeros((end - begin, self.ngeom), dtype=POSE_DTYPE)
+        c = np.zeros((end - begin, self.ncam), dtype=POSE_DTYPE)
+        self._check(self.L.pippin_hip_poses(self.h, begin, end, _p(g), g.size, _p(c), c.size))
+        return g, c
+
+    def param_width(self, p):
+        return {"geom_size": 3, "actuator_bias": 3, "body_inertia": 3, "body_ipos": 3,
+                "geom_rgba": 4, "qpos0": self.nq}.get(p, 1)
+
+    def set_param(self, p, pid, begin, end, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        rc = self.L.pippin_hip_set_param(self.h, PARAM[p], pid, begin, end, _p(v), v.size)
+        if rc == -1:
+            raise ParamError(_err())
+        self._check(rc)
+
+    def get_param(self, p, pid, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        out = np.zeros((end - begin, self.param_width(p)))
+        rc = self.L.pippin_hip_get_param(self.h, PARAM[p], pid, begin, end, _p(out), out.size)
+        if rc == -1:
+            raise ParamError(_err())
+        self._check(rc)
+        return out
+
+    def appearance(self, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        out = np.zeros((end - begin, self.ngeom), dtype=APPEARANCE_DTYPE)
+        self._check(self.L.pippin_hip_appearance(self.h, begin, end, _p(out), out.size))
+        return out
+
+    # ---- diagnostics ----
+    def ncon(self, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        out = np.zeros(end - begin, dtype=np.int32)
+        self._check(self.L.pippin_hip_ncon(self.h, begin, end, _p(out), out.size))
+        return out
+
+    def contacts(self, env):
+        n = C.c_int32(0)
+        buf = np.zeros(512, dtype=CONTACT_DTYPE)
+        self._check(self.L.pippin_hip_contacts(self.h, env, _p(buf), len(buf), C.byref(n)))
+        return buf[: n.value].copy()
+
+    def solver_iterations(self, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        out = np.zeros(end - begin, dtype=np.int32)
+        self._check(self.L.pippin_hip_solver_iterations(self.h, begin, end, _p(out), out.size))
+        return out
+
+    def mass_bias(self, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        qm = np.zeros((end - begin, self.nv, self.nv))
+        bias = np.zeros((end - begin, self.nv))
+        self._check(self.L.pippin_hip_mass_bias(self.h, begin, end, _p(qm), qm.size, _p(bias), bias.size))
+        return qm, bias
+
+    def invweight(self, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        dof = np.zeros((end - begin, self.nv))
+        body = np.zeros((end - begin, self.nbody, 2))
+        self._check(self.L.pippin_hip_invweight(self.h, begin, end, _p(dof), dof.size, _p(body), body.size))
+        return dof, body
+
+    def collision_pairs(self, env=0):
+        n = C.c_int32(0)
+        buf = np.zeros((8192, 2), dtype=np.int32)
+        self._check(self.L.pippin_hip_collision_pairs(self.h, env, _p(buf), len(buf), C.byref(n)))
+        return buf[: n.value].copy()
+
+    # ---- device-resident paths ----
+    def state_ptr(self, field):
+        p = C.c_void_p()
+        self._check(self.L.pippin_hip_state_device(self.h, FIELD[field], C.byref(p)))
+        return p.value
+
+    def rollout_device(self, ctrl_ptr: int, qpos_out_ptr: int | None, nstep: int, begin=0, end=None):
+        """Batch::rollout with device pointers (e.g. torch.cuda tensors' data_ptr())."""
+        end = self.n if end is None else end
+        self._check(self.L.pippin_hip_rollout_device(self.h, begin, end, nstep, C.c_void_p(ctrl_ptr),
+                                                      C.c_void_p(qpos_out_ptr) if qpos_out_ptr else None))
