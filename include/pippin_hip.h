@@ -219,6 +219,11 @@ int pippin_hip_invweight(const ph_sim* s, int64_t begin, int64_t end, double* do
 /* Number of candidate collision pairs and their (g1, g2) list (compile-time filtering). */
 int pippin_hip_collision_pairs(const ph_sim* s, int64_t env, int32_t* pairs, int32_t cap, int32_t* n);
 
+/* Device arithmetic check: for each x[i], out[4i..4i+3] = (sin x, cos x, sqrt|x|, 1/x)
+ * as the kernels compute them (sin/cos correctly rounded). No counterpart in the
+ * reference; it lets callers confirm the device math matches the host bit for bit. */
+int pippin_hip_selftest_math(const double* x, int64_t n, double* out, int32_t device);
+
 /* ---- device-resident rollout (Batch::rollout, crates/pippin/src/batch.rs:46-59) ---- */
 /* All pointers are DEVICE pointers: ctrl (envs x nstep x nu), qpos_out (envs x nstep x nq)
  * for envs [begin, end). Runs on the library's stream and synchronises. */

@@ -350,13 +350,17 @@ PHD void collide(const DModel& M, const DState& S, Env& E) {
       off[p] = o;
       o += cnt[p];
     }
+    if (o > M.ncmax) {  // only reachable with a user-lowered contact capacity
+      isc[7] |= ST_CONTACT_OVERFLOW;
+      o = M.ncmax;
+    }
     isc[0] = o;  // ncon
   });
   Real* con = S.con + E.e * (int64_t)M.ncmax * CW;
   int* cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
   X::par(M.npair, [&](int p) {
     int n = cnt[p];
-    for (int h = 0; h < n; h++) {
+    for (int h = 0; h < n && off[p] + h < M.ncmax; h++) {
       const Real* src = stage + ((int64_t)p * MAXHIT + h) * CW;
       Real* dst = con + (int64_t)(off[p] + h) * CW;
       for (int k = 0; k < CW; k++) dst[k] = src[k];
@@ -820,6 +824,7 @@ PHD bool step_env(const DModel& M, const DState& S, Env& E) {
   rne<X>(M, E);
   passive_actuation<X>(M, E);
   collide<X>(M, S, E);
+  if (isc[7] & ST_CONTACT_OVERFLOW) return false;
   Real* smooth = LD(smooth);
   Real* acc = LD(acc);
   Real* a = LD(a);

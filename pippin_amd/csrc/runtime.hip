@@ -102,6 +102,19 @@ __global__ void __launch_bounds__(64) k_setconst(DModel M, const int* vars, Real
   set_const_env<DevX>(M, E, dof_iw + (int64_t)E.var * M.nv, body_iw + (int64_t)E.var * M.nbody * 2);
 }
 
+// Device math check: correctly rounded sin/cos, sqrt and reciprocal per input.
+__global__ void __launch_bounds__(64) k_selftest_math(const Real* x, int64_t n, Real* out) {
+  int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (i < n) {
+    Real s, c;
+    cr_sincos(x[i], &s, &c);
+    out[4 * i] = s;
+    out[4 * i + 1] = c;
+    out[4 * i + 2] = sqrt(fabs(x[i]));
+    out[4 * i + 3] = 1.0 / x[i];
+  }
+}
+
 // ---------------------------------------------------------------- host runtime
 static thread_local std::string g_err;
 void ph_set_error(const std::string& msg) { g_err = msg; }
@@ -919,5 +932,20 @@ extern "C" int pippin_hip_collision_pairs(const ph_sim* s, int64_t env, int32_t*
     cnt++;
   }
   *n = cnt;
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_selftest_math(const double* x, int64_t n, double* out, int32_t device) {
+  if (n <= 0) return PH_OK;
+  if (!x || !out) return fail(PH_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(device));
+  DBuf<double> dx, dout;
+  HIPCHK(dx.alloc(n));
+  HIPCHK(dout.alloc(4 * n));
+  HIPCHK(hipMemcpy(dx.p, x, n * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_selftest_math, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, 0, dx.p, n, dout.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(out, dout.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
   return PH_OK;
 }

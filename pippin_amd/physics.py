@@ -72,6 +72,7 @@ def lib():
             "pippin_hip_state_device": ([vp, i32, C.POINTER(vp)], C.c_int),
             "pippin_hip_stream": ([vp], vp),
             "pippin_hip_last_error": ([], C.c_char_p),
+            "pippin_hip_selftest_math": ([vp, i64, vp, i32], C.c_int),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
@@ -103,6 +104,15 @@ def load_model(path_or_xml: str | Path, is_xml: bool = False) -> dict:
         return arrays_from_desc(L.pippin_hip_model_desc(h).contents)
     finally:
         L.pippin_hip_model_free(h)
+
+
+def selftest_math(x, device: int = 0):
+    """(sin, cos, sqrt|x|, 1/x) of each x as computed on the device."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((x.size, 4))
+    if lib().pippin_hip_selftest_math(_p(x), x.size, _p(out), device) != 0:
+        raise HipError(_err())
+    return out
 
 
 class HipPhysics:

@@ -1,0 +1,107 @@
+"""HIP kernels vs the oracle on identical inputs (needs a GPU).
+
+Bars (from the target): contact counts and pair indices bit-exact per step,
+M(q) and c(q, qdot) within 1e-12 relative, qpos within 1e-12 contact-free and
+1e-9 through contact. The kernels keep the reference's operation order and the
+oracle and device share correctly rounded sin/cos, so the tests check for
+exact equality first and report the tolerance-based bar alongside.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mjcf
+from oracle.pyoracle import OracleSim, cr_sincos
+from pippin_amd.physics import HipPhysics, load_model, selftest_math
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_device_math_matches_host():
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-20, 20, 20000), rng.uniform(-1e-3, 1e-3, 2000),
+                        [0.0, -0.0, 1e-300, math.pi / 4, math.pi / 2, 1.0, -2.5, 3.0]])
+    say("device math on", x.size, "inputs")
+    out = selftest_math(x)
+    s, c = cr_sincos(x)
+    assert np.array_equal(out[:, 0], s) and np.array_equal(out[:, 1], c), "cr_sincos differs from the oracle's"
+    assert np.array_equal(out[:, 2], np.sqrt(np.abs(x))), "device sqrt is not correctly rounded"
+    with np.errstate(divide="ignore"):
+        assert np.array_equal(out[:, 3], 1.0 / x), "device division is not correctly rounded"
+
+
+def _ant(assets, n):
+    arr = mjcf.load_file(assets / "ant.xml")
+    return arr, OracleSim(arr, n), HipPhysics(arr, n)
+
+
+def _perturbed_qpos(sim, rng, n, scale=0.01):
+    q = sim.get("qpos")
+    q = q + rng.normal(size=q.shape) * scale
+    for i in range(n):
+        q[i, 3:7] /= np.linalg.norm(q[i, 3:7])
+    return q
+
+
+def test_ant_mass_and_bias_match(assets):
+    n = 64
+    arr, o, g = _ant(assets, n)
+    rng = np.random.default_rng(0)
+    q = _perturbed_qpos(o, rng, n, 0.3)
+    v = rng.normal(size=(n, 14))
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+        s.set("qvel", 0, n, v)
+    say("ant mass/bias", n, "envs")
+    m1, b1 = o.mass_bias()
+    m2, b2 = g.mass_bias()
+    rel_m = np.abs(m1 - m2).max() / np.abs(m1).max()
+    rel_b = np.abs(b1 - b2).max() / np.abs(b1).max()
+    say("M rel err", rel_m, "c rel err", rel_b)
+    assert rel_m <= 1e-12 and rel_b <= 1e-12
+    assert np.array_equal(m1, m2) and np.array_equal(b1, b2)
+    d1, w1 = o.invweight()
+    d2, w2 = g.invweight()
+    assert np.array_equal(d1, d2) and np.array_equal(w1, w2)
+
+
+def test_ant_trajectories_match(assets):
+    """Config 0 shape: qpos0 + N(0, 0.01), ctrl U[-1, 1] per step, Newton, 200 steps."""
+    n = 256
+    arr, o, g = _ant(assets, n)
+    rng = np.random.default_rng(0)
+    q = _perturbed_qpos(o, rng, n)
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    ctrl = rng.uniform(-1, 1, size=(200, n, 8))
+    worst = 0.0
+    for t in range(200):
+        for s in (o, g):
+            s.set("ctrl", 0, n, ctrl[t])
+            s.step(0, n, 1)
+        nc_o, nc_g = o.ncon(), g.ncon()
+        assert np.array_equal(nc_o, nc_g), f"ncon differs at step {t}"
+        if t % 50 == 0:
+            for e in range(0, n, 37):
+                co, cg = o.contacts(e), g.contacts(e)
+                assert np.array_equal(co[["geom1", "geom2", "feature"]], cg[["geom1", "geom2", "feature"]])
+        err = np.abs(o.get("qpos") - g.get("qpos")).max()
+        worst = max(worst, err)
+        assert err <= 1e-9, f"qpos error {err} at step {t}"
+    say("ant 200 steps: worst qpos err", worst, "contacts", int(o.ncon().sum()))
+    assert np.array_equal(o.solver_iterations(), g.solver_iterations())
+    assert worst == 0.0
+
+
+def test_library_loader_matches_oracle_loader(assets):
+    a = load_model(assets / "ant.xml")
+    b = mjcf.load_file(assets / "ant.xml")
+    for k in b:
+        assert np.array_equal(np.asarray(a[k]).reshape(-1), np.asarray(b[k]).reshape(-1).astype(np.asarray(a[k]).dtype)), k
