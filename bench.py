@@ -1,0 +1,189 @@
+"""Throughput of the batched FP64 physics step on one MI355X.
+
+Workload (the headline configuration): Ant (pippin_amd/assets/ant.xml: free torso +
+8 hinge legs, nv = 14, capsules on a ground plane), 16384 envs, 4 substeps per
+action with the control held across them, Newton solver, FP64. Initial state
+qpos0 + N(0, 0.01) (quaternion renormalised), controls i.i.d. U[-1, 1] per env,
+actuator and action, seed 0. One bench "step" is one action: a control update
+followed by 4 physics steps of every env (Physics::set + Physics::step, as
+AsyncEnv drives a backend). `value` counts physics steps: env-steps/s =
+envs x substeps x actions / s. The controls for all actions are resident in HBM
+before the timed region; each action copies its slice device-to-device.
+
+Printed extras: `roofline` for the step kernel (counted FP64 operations per
+env-step of the reference algorithm, measured by the oracle's op-counting build,
+over the kernel time from hipEvents on the library's stream) and `cpu_baseline`
+(the oracle, a C++ restatement of Pippin's CPU engine, stepping a sample of the
+same workload on the host cores with Pippin's chunked thread pool).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "env-steps/s on 1 MI355X (Ant 16k envs, contact) at ≤1e-9 err; Newton iters/step"
+FP64_PEAK_TFLOPS = 78.6
+# reference FP64 ops per env-step on this workload, used only with --no-cpu
+OPS_PER_ENV_STEP_DEFAULT = 25000.0
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def initial_qpos(qpos0: np.ndarray, n: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    q = np.tile(qpos0, (n, 1)) + rng.normal(0.0, 0.01, size=(n, qpos0.size))
+    q[:, 3:7] /= np.linalg.norm(q[:, 3:7], axis=1, keepdims=True)
+    return q
+
+
+def controls(n: int, nu: int, nact: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed + 1)
+    return rng.uniform(-1.0, 1.0, size=(nact, n, nu))
+
+
+def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
+    """Time the oracle on a bounded sample of the same workload on the host cores.
+
+    The same actions as the timed GPU window (after the same warmup), for the
+    first `sample` envs, step-synchronous over the range with Pippin's chunked
+    thread pool. Also counts the FP64 operations of the reference algorithm on
+    that window (op-counting build, 256 envs), the roofline's numerator.
+    """
+    from oracle.pyoracle import OracleSim, ops_per_env_step
+
+    threads = len(os.sched_getaffinity(0))
+    nact, n, nu = ctrl.shape
+    sample = min(n, sample)
+    sim = OracleSim(arrays, sample)
+    sim.set("qpos", 0, sample, q0[:sample])
+    for a in range(warmup):
+        sim.set("ctrl", 0, sample, ctrl[a, :sample])
+        sim.step(0, sample, substeps, threads=threads)
+    t0 = time.perf_counter()
+    for a in range(warmup, nact):
+        sim.set("ctrl", 0, sample, ctrl[a, :sample])
+        sim.step(0, sample, substeps, threads=threads)
+    dt = time.perf_counter() - t0
+    env_steps = sample * substeps * (nact - warmup)
+    ops = ops_per_env_step(arrays, q0[:256], ctrl[:, :256], substeps, skip=warmup)
+    return {
+        "value": env_steps / dt,
+        "unit": "env-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"Ant, envs 0..{sample} of {n}, the {nact - warmup} timed actions x {substeps} substeps "
+                  f"after the same {warmup} warmup actions ({env_steps} env-steps in {dt:.1f} s), "
+                  f"Physics::step over the range, chunk = n/(4 x {threads} threads)",
+    }, ops
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=250, help="timed actions")
+    ap.add_argument("--warmup", type=int, default=25, help="untimed actions")
+    ap.add_argument("--envs", type=int, default=16384)
+    ap.add_argument("--substeps", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-envs", type=int, default=4096, help="envs in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one MI355X and the environments are "
+              f"independent (zero exchange while stepping), so --gpus {args.gpus} is not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pippin_amd.physics import HipPhysics, load_model
+
+    arrays = load_model(ROOT / "pippin_amd" / "assets" / "ant.xml")
+    n, sub = args.envs, args.substeps
+    nact = args.warmup + args.steps
+    sim = HipPhysics(arrays, n)
+    nu = sim.nu
+    q0 = initial_qpos(np.asarray(arrays["qpos0"]), n, args.seed)
+    ctrl = controls(n, nu, nact, args.seed)
+    sim.set("qpos", 0, n, q0)
+    dctrl = torch.from_numpy(ctrl).to("cuda")  # resident in HBM before timing
+    base = dctrl.data_ptr()
+    stride = n * nu * 8
+    log(f"bench: {n} envs x {sub} substeps, {args.warmup} warmup + {args.steps} timed actions")
+
+    for a in range(args.warmup):
+        sim.set_device("ctrl", 0, n, base + a * stride)
+        sim.step(0, n, sub)
+    torch.cuda.synchronize()
+    sim.kernel_timing(reset=True)
+    t0 = time.perf_counter()
+    for a in range(args.warmup, nact):
+        sim.set_device("ctrl", 0, n, base + a * stride)
+        sim.step(0, n, sub)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kms, launches = sim.kernel_timing(reset=True)
+    env_steps = n * sub * args.steps
+    value = env_steps / dt
+    log(f"timed {args.steps} actions in {dt:.3f} s; kernel {kms / launches:.3f} ms/launch")
+
+    # Newton iterations per step and contacts, over a few more (untimed) actions
+    iters, ncon = [], []
+    for a in range(nact - min(20, nact), nact):
+        sim.set_device("ctrl", 0, n, base + a * stride)
+        for _ in range(sub):
+            sim.step(0, n, 1)
+            iters.append(sim.solver_iterations())
+            ncon.append(sim.ncon())
+    iters = np.concatenate(iters)
+    ncon = np.concatenate(ncon)
+
+    cb, ops = (None, OPS_PER_ENV_STEP_DEFAULT) if args.no_cpu else cpu_baseline(
+        arrays, q0, ctrl, sub, args.warmup, args.cpu_envs)
+    # roofline of the step kernel: FP64 operations of the reference algorithm per env-step
+    # (op-counting oracle on the same window) over the kernel's own time
+    kernel_s = kms / 1e3 / launches
+    achieved = ops * n * sub / kernel_s / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "env-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "Ant (assets/ant.xml) 16384 envs x 4 substeps per action, Newton, FP64, "
+                               "ctrl U[-1,1] per action, qpos0 + N(0,0.01)",
+                   "envs": n, "substeps": sub, "actions": args.steps},
+        "newton_iters_per_step": {"mean": float(iters.mean()), "p50": float(np.percentile(iters, 50)),
+                                  "p99": float(np.percentile(iters, 99))},
+        "contacts_per_env": {"mean": float(ncon.mean()), "max": int(ncon.max())},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": f"FP64 VALU work (MI355X FP64 peak, vector = matrix); {ops:.0f} reference FP64 ops "
+                             f"per env-step; step kernel {kernel_s * 1e3:.3f} ms per launch "
+                             f"({n} envs x {sub} steps)"},
+    }
+    if cb is not None:
+        line["cpu_baseline"] = cb
+        line["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

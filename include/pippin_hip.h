@@ -231,6 +231,12 @@ int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, int64_t nst
                               double* qpos_out);
 /* Device pointers of the resident state (envs x width), for zero-copy callers. */
 int pippin_hip_state_device(ph_sim* s, int32_t field, double** ptr);
+/* Copy `field` (PH_FIELD_QVEL or PH_FIELD_CTRL) for envs [begin, end) from a DEVICE
+ * buffer, ordered on the library stream before the next step (no host sync). */
+int pippin_hip_set_device(ph_sim* s, int32_t field, int64_t begin, int64_t end, const double* src);
+/* Accumulated device time of the step kernel (hipEvents around each launch on the
+ * library stream) and the number of launches since the last reset. */
+int pippin_hip_kernel_timing(ph_sim* s, int32_t reset, double* total_ms, int64_t* launches);
 /* Stream the library launches on (a hipStream_t). */
 void* pippin_hip_stream(ph_sim* s);
 

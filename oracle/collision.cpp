@@ -255,7 +255,7 @@ static void capsule_box(const GeomPose& a, const GeomPose& b, Real margin, std::
   std::vector<Real> ts = {tmin};
   const Real ends[2] = {0.0, 1.0};
   for (Real t : ends)
-    if (std::fabs(t - tmin) > 0.1 && f(t) < r + margin) ts.push_back(t);
+    if (r_fabs(t - tmin) > 0.1 && f(t) < r + margin) ts.push_back(t);
   for (Real t : ts) sphere_box(p0 + (p1 - p0) * t, r, b, margin, out);
 }
 
@@ -267,7 +267,7 @@ static std::vector<Vec3> clip(const std::vector<Vec3>& poly, Vec3 pn, Real off) 
     Vec3 a = poly[i], b = poly[(i + 1) % n];
     Real da = dot(pn, a) - off, db = dot(pn, b) - off;
     if (da <= 0.0) out.push_back(a);
-    if ((da < 0.0) != (db < 0.0) && std::fabs(da - db) > 1e-15) out.push_back(a + (b - a) * (da / (da - db)));
+    if ((da < 0.0) != (db < 0.0) && r_fabs(da - db) > 1e-15) out.push_back(a + (b - a) * (da / (da - db)));
   }
   return out;
 }
@@ -311,7 +311,7 @@ static void box_box(const GeomPose& a, const GeomPose& b, Real margin, std::vect
   Vec3 ax[3] = {col(a.mat, 0), col(a.mat, 1), col(a.mat, 2)};
   Vec3 bx[3] = {col(b.mat, 0), col(b.mat, 1), col(b.mat, 2)};
   auto proj = [](const Vec3* axes, Vec3 h, Vec3 l) {
-    return h[0] * std::fabs(dot(axes[0], l)) + h[1] * std::fabs(dot(axes[1], l)) + h[2] * std::fabs(dot(axes[2], l));
+    return h[0] * r_fabs(dot(axes[0], l)) + h[1] * r_fabs(dot(axes[1], l)) + h[2] * r_fabs(dot(axes[2], l));
   };
   bool has_best = false, has_face = false;
   Real best_o = 0, face_o = 0;
@@ -327,7 +327,7 @@ static void box_box(const GeomPose& a, const GeomPose& b, Real margin, std::vect
       if (n < 1e-6) continue;
       l = c * (1.0 / n);
     }
-    Real overlap = proj(ax, a.size, l) + proj(bx, b.size, l) - std::fabs(dot(d, l));
+    Real overlap = proj(ax, a.size, l) + proj(bx, b.size, l) - r_fabs(dot(d, l));
     if (overlap < -margin) return;
     if (dot(d, l) < 0.0) l = -l;
     if (!has_best || overlap < best_o) {
@@ -378,8 +378,8 @@ static void box_box(const GeomPose& a, const GeomPose& b, Real margin, std::vect
   Real bestdot = INFINITY;
   for (int mm = 0; mm < 3; mm++) {
     Real dt = dot(iaxes[mm], nref);
-    if (-std::fabs(dt) < bestdot) {
-      bestdot = -std::fabs(dt);
+    if (-r_fabs(dt) < bestdot) {
+      bestdot = -r_fabs(dt);
       j = mm;
     }
   }

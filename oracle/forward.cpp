@@ -197,14 +197,14 @@ void collide(const Model& m, Data& d) {
   for (size_t g = 0; g < m.ngeom(); g++) {
     Vec3 c = m.geom_aabb[g].first, h = m.geom_aabb[g].second;
     const Mat3& r = d.geom_xmat[g];
-    if (!std::isfinite(h[0])) {
+    if (!r_isfinite(h[0])) {
       aabb[g] = {v3zero(), h};
       continue;
     }
     Vec3 center = d.geom_xpos[g] + mul_vec(r, c);
-    Vec3 half = V3(std::fabs(r[0]) * h[0] + std::fabs(r[1]) * h[1] + std::fabs(r[2]) * h[2],
-                   std::fabs(r[3]) * h[0] + std::fabs(r[4]) * h[1] + std::fabs(r[5]) * h[2],
-                   std::fabs(r[6]) * h[0] + std::fabs(r[7]) * h[1] + std::fabs(r[8]) * h[2]);
+    Vec3 half = V3(r_fabs(r[0]) * h[0] + r_fabs(r[1]) * h[1] + r_fabs(r[2]) * h[2],
+                   r_fabs(r[3]) * h[0] + r_fabs(r[4]) * h[1] + r_fabs(r[5]) * h[2],
+                   r_fabs(r[6]) * h[0] + r_fabs(r[7]) * h[1] + r_fabs(r[8]) * h[2]);
     aabb[g] = {center, half};
   }
   std::vector<Hit> hits;
@@ -213,7 +213,7 @@ void collide(const Model& m, Data& d) {
     Vec3 c1 = aabb[g1].first, h1 = aabb[g1].second, c2 = aabb[g2].first, h2 = aabb[g2].second;
     bool skip = false;
     for (int k = 0; k < 3 && !skip; k++)
-      if (std::fabs(c1[k] - c2[k]) > h1[k] + h2[k] + margin) skip = true;
+      if (r_fabs(c1[k] - c2[k]) > h1[k] + h2[k] + margin) skip = true;
     if (skip) continue;
     GeomPose p1 = {m.geom_type[g1], d.geom_xpos[g1], d.geom_xmat[g1], m.geom_size[g1]};
     GeomPose p2 = {m.geom_type[g2], d.geom_xpos[g2], d.geom_xmat[g2], m.geom_size[g2]};

@@ -12,7 +12,23 @@
 
 namespace orc {
 
+#ifdef ORC_OPCOUNT
+}  // namespace orc
+#include "opcount.hpp"
+namespace orc {
+#else
 typedef double Real;
+static inline Real r_sqrt(Real x) { return std::sqrt(x); }
+static inline Real r_fabs(Real x) { return std::fabs(x); }
+static inline Real r_pow(Real a, Real b) { return std::pow(a, b); }
+static inline bool r_isfinite(Real x) { return std::isfinite(x); }
+static inline bool r_isnan(Real x) { return std::isnan(x); }
+static inline Real r_copysign(Real a, Real b) { return std::copysign(a, b); }
+static inline Real r_fmax(Real a, Real b) { return std::fmax(a, b); }
+static inline Real r_fmin(Real a, Real b) { return std::fmin(a, b); }
+static inline double r_val(Real x) { return x; }
+static inline void r_count(int) {}
+#endif
 
 // Rust's `impl Sum for f64` folds from -0.0; `rsum_start()` keeps that sign rule.
 static inline Real rsum_start() { return -0.0; }
@@ -24,10 +40,10 @@ static inline Real rclamp(Real x, Real lo, Real hi) {
   return x;
 }
 // Rust f64::max / f64::min (IEEE maxNum/minNum: a NaN operand yields the other).
-static inline Real rmax(Real a, Real b) { return std::fmax(a, b); }
-static inline Real rmin(Real a, Real b) { return std::fmin(a, b); }
+static inline Real rmax(Real a, Real b) { return r_fmax(a, b); }
+static inline Real rmin(Real a, Real b) { return r_fmin(a, b); }
 // Rust f64::signum: +1 for +0.0 and positives, -1 for -0.0 and negatives.
-static inline Real rsignum(Real x) { return std::isnan(x) ? x : std::copysign(1.0, x); }
+static inline Real rsignum(Real x) { return r_isnan(x) ? x : r_copysign(1.0, x); }
 
 struct Vec3 {
   Real v[3];
@@ -49,20 +65,20 @@ static inline Vec3 cross(Vec3 a, Vec3 o) {
   return V3(a[1] * o[2] - a[2] * o[1], a[2] * o[0] - a[0] * o[2], a[0] * o[1] - a[1] * o[0]);
 }
 static inline Real norm2(Vec3 a) { return dot(a, a); }
-static inline Real norm(Vec3 a) { return std::sqrt(norm2(a)); }
+static inline Real norm(Vec3 a) { return r_sqrt(norm2(a)); }
 // math.rs:40-47
 static inline Vec3 normalized(Vec3 a) {
   Real n = norm(a);
   if (n > 1e-15) return a * (1.0 / n);
   return v3zero();
 }
-static inline Vec3 vabs(Vec3 a) { return V3(std::fabs(a[0]), std::fabs(a[1]), std::fabs(a[2])); }
+static inline Vec3 vabs(Vec3 a) { return V3(r_fabs(a[0]), r_fabs(a[1]), r_fabs(a[2])); }
 static inline Vec3 mul_elem(Vec3 a, Vec3 o) { return V3(a[0] * o[0], a[1] * o[1], a[2] * o[2]); }
 static inline Vec3 vmin(Vec3 a, Vec3 o) { return V3(rmin(a[0], o[0]), rmin(a[1], o[1]), rmin(a[2], o[2])); }
 static inline Vec3 vmax(Vec3 a, Vec3 o) { return V3(rmax(a[0], o[0]), rmax(a[1], o[1]), rmax(a[2], o[2])); }
 // math.rs:65-70
 static inline void tangents(Vec3 n, Vec3* t1, Vec3* t2) {
-  Vec3 a = std::fabs(n[0]) < 0.57 ? V3(1, 0, 0) : V3(0, 1, 0);
+  Vec3 a = r_fabs(n[0]) < 0.57 ? V3(1, 0, 0) : V3(0, 1, 0);
   *t1 = normalized(cross(n, a));
   *t2 = cross(n, *t1);
 }
@@ -131,18 +147,18 @@ static inline Mat3 mrotate(const Mat3& self, const Mat3& r) { return mul_mat(mul
 // within ~2^-100 relative of the exact value before the final rounding, so it
 // equals the exactly rounded value except at ties closer than that.
 struct DD {
-  Real hi, lo;
+  double hi, lo;
 };
-static inline DD dd_fast_two_sum(Real a, Real b) {
-  Real s = a + b;
+static inline DD dd_fast_two_sum(double a, double b) {
+  double s = a + b;
   DD r;
   r.hi = s;
   r.lo = b - (s - a);
   return r;
 }
-static inline DD dd_two_sum(Real a, Real b) {
-  Real s = a + b;
-  Real bb = s - a;
+static inline DD dd_two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
   DD r;
   r.hi = s;
   r.lo = (a - (s - bb)) + (b - bb);
@@ -151,39 +167,39 @@ static inline DD dd_two_sum(Real a, Real b) {
 static inline DD dd_add(DD a, DD b) {
   DD s = dd_two_sum(a.hi, b.hi);
   DD t = dd_two_sum(a.lo, b.lo);
-  Real e = s.lo + t.hi;
+  double e = s.lo + t.hi;
   DD u = dd_fast_two_sum(s.hi, e);
   e = u.lo + t.lo;
   return dd_fast_two_sum(u.hi, e);
 }
 static inline DD dd_mul(DD a, DD b) {
-  Real p = a.hi * b.hi;
-  Real e = std::fma(a.hi, b.hi, -p);
+  double p = a.hi * b.hi;
+  double e = std::fma(a.hi, b.hi, -p);
   e = e + (a.hi * b.lo + a.lo * b.hi);
   return dd_fast_two_sum(p, e);
 }
-static inline DD dd_from(Real hi, Real lo) {
+static inline DD dd_from(double hi, double lo) {
   DD r;
   r.hi = hi;
   r.lo = lo;
   return r;
 }
-static inline void cr_sincos(Real x, Real* sout, Real* cout) {
+static inline void cr_sincos(double x, double* sout, double* cout) {
   if (!std::isfinite(x)) {
     *sout = x - x;
     *cout = x - x;
     return;
   }
-  const Real P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
-  Real k = std::rint(x * 0.6366197723675814);
+  const double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
+  double k = std::rint(x * 0.6366197723675814);
   DD r = dd_from(x, 0.0);
-  Real h = k * P1;
+  double h = k * P1;
   r = dd_add(r, dd_from(-h, -std::fma(k, P1, -h)));
   h = k * P2;
   r = dd_add(r, dd_from(-h, -std::fma(k, P2, -h)));
   r = dd_add(r, dd_from(-(k * P3), 0.0));
   DD r2 = dd_mul(r, r);
-  const Real S[15][2] = {{1.0, 0.0},
+  const double S[15][2] = {{1.0, 0.0},
                          {-0.16666666666666666, -9.25185853854297e-18},
                          {0.008333333333333333, 1.1564823173178714e-19},
                          {-0.0001984126984126984, -1.7209558293420705e-22},
@@ -198,7 +214,7 @@ static inline void cr_sincos(Real x, Real* sout, Real* cout) {
                          {6.446950284384474e-26, 0.0},
                          {-9.183689863795546e-29, 0.0},
                          {1.1309962886447716e-31, 0.0}};
-  const Real Cc[15][2] = {{1.0, 0.0},
+  const double Cc[15][2] = {{1.0, 0.0},
                           {-0.5, 0.0},
                           {0.041666666666666664, 2.3129646346357427e-18},
                           {-0.001388888888888889, 5.300543954373577e-20},
@@ -213,7 +229,7 @@ static inline void cr_sincos(Real x, Real* sout, Real* cout) {
                           {1.6117375710961184e-24, 0.0},
                           {-2.4795962632247976e-27, 0.0},
                           {3.279889237069838e-30, 0.0}};
-  Real ts = S[14][0], tc = Cc[14][0];
+  double ts = S[14][0], tc = Cc[14][0];
   for (int i = 13; i >= 9; i--) {
     ts = ts * r2.hi + S[i][0];
     tc = tc * r2.hi + Cc[i][0];
@@ -224,7 +240,7 @@ static inline void cr_sincos(Real x, Real* sout, Real* cout) {
     pc = dd_add(dd_mul(pc, r2), dd_from(Cc[i][0], Cc[i][1]));
   }
   DD s = dd_mul(ps, r);
-  Real sv = s.hi, cv = pc.hi;
+  double sv = s.hi, cv = pc.hi;
   long q = (long)k;
   q = ((q % 4) + 4) % 4;
   if (q == 0) {
@@ -252,14 +268,16 @@ static inline Quat Q4(Real w, Real x, Real y, Real z) { Quat r = {{w, x, y, z}};
 static inline Quat qidentity() { return Q4(1, 0, 0, 0); }
 static inline Quat from_axis_angle(Vec3 axis, Real angle) {
   Vec3 a = normalized(axis);
-  Real s, c;
-  cr_sincos(0.5 * angle, &s, &c);
+  double sd, cd;
+  cr_sincos(r_val(0.5 * angle), &sd, &cd);
+  r_count(2);  // one sin and one cos, as the reference's libm sin_cos
+  Real s = sd, c = cd;
   return Q4(c, a[0] * s, a[1] * s, a[2] * s);
 }
 static inline Quat qnormalized(Quat q) {
   Real s = rsum_start();
   for (int i = 0; i < 4; i++) s += q[i] * q[i];
-  Real n = std::sqrt(s);
+  Real n = r_sqrt(s);
   if (n < 1e-15) return qidentity();
   return Q4(q[0] / n, q[1] / n, q[2] / n, q[3] / n);
 }
@@ -283,16 +301,16 @@ static inline Quat from_mat(const Mat3& m) {
   Real tr = m[0] + m[4] + m[8];
   Quat q;
   if (tr > 0.0) {
-    Real s = std::sqrt(tr + 1.0) * 2.0;
+    Real s = r_sqrt(tr + 1.0) * 2.0;
     q = Q4(0.25 * s, (m[7] - m[5]) / s, (m[2] - m[6]) / s, (m[3] - m[1]) / s);
   } else if (m[0] > m[4] && m[0] > m[8]) {
-    Real s = std::sqrt(1.0 + m[0] - m[4] - m[8]) * 2.0;
+    Real s = r_sqrt(1.0 + m[0] - m[4] - m[8]) * 2.0;
     q = Q4((m[7] - m[5]) / s, 0.25 * s, (m[1] + m[3]) / s, (m[2] + m[6]) / s);
   } else if (m[4] > m[8]) {
-    Real s = std::sqrt(1.0 + m[4] - m[0] - m[8]) * 2.0;
+    Real s = r_sqrt(1.0 + m[4] - m[0] - m[8]) * 2.0;
     q = Q4((m[2] - m[6]) / s, (m[1] + m[3]) / s, 0.25 * s, (m[5] + m[7]) / s);
   } else {
-    Real s = std::sqrt(1.0 + m[8] - m[0] - m[4]) * 2.0;
+    Real s = r_sqrt(1.0 + m[8] - m[0] - m[4]) * 2.0;
     q = Q4((m[3] - m[1]) / s, (m[2] + m[6]) / s, (m[5] + m[7]) / s, 0.25 * s);
   }
   q = qnormalized(q);
@@ -401,7 +419,7 @@ static inline bool cholesky_block(Real* a, size_t stride, size_t s0, size_t e) {
     size_t len = j - s0;
     Real d = a[j * stride + j] - sdot_n(rj, rj, len);
     if (d <= 0.0) return false;
-    d = std::sqrt(d);
+    d = r_sqrt(d);
     a[j * stride + j] = d;
     Real inv = 1.0 / d;
     for (size_t i = j + 1; i < e; i++) {

@@ -71,7 +71,7 @@ Real rbound(const Model& m, size_t g) {
     case Sphere: return s[0];
     case Capsule: return s[0] + s[1];
     case Box: return norm(s);
-    case Cylinder: return std::sqrt(s[0] * s[0] + s[1] * s[1]);
+    case Cylinder: return r_sqrt(s[0] * s[0] + s[1] * s[1]);
     default: return 0.0;
   }
 }

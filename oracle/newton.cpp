@@ -12,9 +12,9 @@ typedef unsigned __int128 u128;
 // newton.rs:52-66
 Real impedance(const Real solimp[5], Real pos) {
   Real dmin = solimp[0], dmax = solimp[1], width = solimp[2], mid = solimp[3], power = solimp[4];
-  Real x = rmin(std::fabs(pos) / rmax(width, 1e-12), 1.0);
+  Real x = rmin(r_fabs(pos) / rmax(width, 1e-12), 1.0);
   if (x >= 1.0 || dmin == dmax) return dmax;
-  auto pw = [](Real b, Real e) { return e == 2.0 ? b * b : (e == 1.0 ? b : std::pow(b, e)); };
+  auto pw = [](Real b, Real e) { return e == 2.0 ? b * b : (e == 1.0 ? b : r_pow(b, e)); };
   Real y = x <= mid ? pw(x, power) / pw(mid, power - 1.0) : 1.0 - pw(1.0 - x, power) / pw(1.0 - mid, power - 1.0);
   return rclamp(dmin + y * (dmax - dmin), 1e-4, 0.9999);
 }
@@ -174,7 +174,7 @@ static size_t newton_dense(const Workspace& w, Real* a, const Real* warm, size_t
   w.mul_m(a0, tmp);
   Real s0 = rsum_start();
   for (size_t k = 0; k < nv; k++) s0 += tmp[k] * tmp[k];
-  Real scale = 1.0 + std::sqrt(s0);
+  Real scale = 1.0 + r_sqrt(s0);
 
   size_t iters = 0;
   u128 prev_active = ~(u128)0;
@@ -195,7 +195,7 @@ static size_t newton_dense(const Workspace& w, Real* a, const Real* warm, size_t
     }
     Real gs = rsum_start();
     for (size_t k = 0; k < nv; k++) gs += grad[k] * grad[k];
-    Real gnorm = std::sqrt(gs);
+    Real gnorm = r_sqrt(gs);
     if (gnorm < tolerance * scale || (nr <= 128 && active == prev_active && iters > 1)) break;
     prev_active = active;
 
@@ -239,17 +239,17 @@ static size_t newton_dense(const Workspace& w, Real* a, const Real* warm, size_t
     Real lo = 0.0, hi = INFINITY, al = 1.0;
     Real d1_0, d2_0;
     deriv(0.0, &d1_0, &d2_0);
-    Real tol = 1e-12 * rmax(std::fabs(d1_0), 1e-30);
+    Real tol = 1e-12 * rmax(r_fabs(d1_0), 1e-30);
     for (int it = 0; it < 50; it++) {
       Real d1, d2;
       deriv(al, &d1, &d2);
       if (ls_steps) (*ls_steps)++;
-      if (std::fabs(d1) <= tol) break;
+      if (r_fabs(d1) <= tol) break;
       if (d1 < 0.0) lo = al;
       else hi = al;
       Real next = al - d1 / rmax(d2, 1e-30);
-      if (next <= lo || next >= hi) next = std::isfinite(hi) ? 0.5 * (lo + hi) : 2.0 * rmax(al, lo);
-      if (std::fabs(next - al) <= 1e-14 * rmax(std::fabs(al), 1.0)) {
+      if (next <= lo || next >= hi) next = r_isfinite(hi) ? 0.5 * (lo + hi) : 2.0 * rmax(al, lo);
+      if (r_fabs(next - al) <= 1e-14 * rmax(r_fabs(al), 1.0)) {
         al = next;
         break;
       }
@@ -440,7 +440,7 @@ void pgs_solve(const Model& m, Data& d, Real* v, std::vector<Real>& v_pos) {
     Real q = d.qpos[m.jnt_qposadr[j]];
     size_t da = m.jnt_dofadr[j];
     Real lo = m.jnt_range_lo[j], hi = m.jnt_range_hi[j];
-    Real reach = opt.contact_margin + 2.0 * h * std::fabs(v[da]);
+    Real reach = opt.contact_margin + 2.0 * h * r_fabs(v[da]);
     Real seps[2] = {q - lo, hi - q}, signs[2] = {1.0, -1.0};
     for (int s = 0; s < 2; s++) {
       if (seps[s] < reach) {
@@ -553,7 +553,7 @@ void pgs_solve(const Model& m, Data& d, Real* v, std::vector<Real>& v_pos) {
       }
       Real dl = nw - lambda[r];
       if (dl != 0.0) {
-        max_dv = rmax(max_dv, std::fabs(dl) / row.inv_a);
+        max_dv = rmax(max_dv, r_fabs(dl) / row.inv_a);
         const Real* mr = mj.data() + r * nv;
         for (size_t k = 0; k < nv; k++) v[k] += mr[k] * dl;
         lambda[r] = nw;

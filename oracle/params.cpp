@@ -67,7 +67,8 @@ std::vector<Real> get_param(const Model& m, int p, size_t id) {
 }
 
 // Rust `f64 as u32` saturates: NaN -> 0, negatives -> 0, > u32::MAX -> u32::MAX.
-static uint32_t sat_u32(Real x) {
+static uint32_t sat_u32(Real xr) {
+  double x = r_val(xr);
   if (!(x > 0.0)) return 0;
   if (x >= 4294967295.0) return 4294967295u;
   return (uint32_t)x;
@@ -87,7 +88,7 @@ bool set_param(Model& m, int p, size_t id, const Real* v, size_t n, std::string*
     return false;
   }
   for (size_t i = 0; i < n; i++)
-    if (!std::isfinite(v[i])) {
+    if (!r_isfinite(v[i])) {
       snprintf(buf, sizeof buf, "%s: values must be finite", param_name(p));
       *err = buf;
       return false;

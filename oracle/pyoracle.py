@@ -308,3 +308,50 @@ def collide(ta, pa, ma, sa, tb, pb, mb, sb, margin=1e-3):
     out = np.zeros(16, dtype=CONTACT_DTYPE)
     n = lib().orc_collide(ta, _p(pa), _p(ma), _p(sa), tb, _p(pb), _p(mb), _p(sb), float(margin), _p(out), 16)
     return out[:n].copy()
+
+
+# ---- op-counting build (liborc_count.so) ----
+_clib = None
+
+
+def count_lib():
+    global _clib
+    if _clib is None:
+        path = HERE / "_build" / "liborc_count.so"
+        if not path.exists():
+            build()
+        L = C.CDLL(str(path))
+        L.orcc_create.restype = C.c_void_p
+        L.orcc_create.argtypes = [C.c_void_p, C.c_int64]
+        L.orcc_destroy.argtypes = [C.c_void_p]
+        L.orcc_set.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_void_p]
+        L.orcc_step.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.POINTER(C.c_ulonglong)]
+        _clib = L
+    return _clib
+
+
+def ops_per_env_step(arrays: dict, qpos: np.ndarray, ctrl_seq: np.ndarray, substeps: int, skip: int = 0) -> float:
+    """FP64 operations of the reference step, averaged over envs and steps.
+
+    qpos: (n, nq) initial state; ctrl_seq: (actions, n, nu), held for `substeps` steps
+    each. The first `skip` actions are stepped but not counted.
+    """
+    L = count_lib()
+    d = Desc(arrays)
+    n = qpos.shape[0]
+    h = L.orcc_create(d.ptr(), n)
+    try:
+        q = np.ascontiguousarray(qpos, dtype=np.float64)
+        L.orcc_set(h, FIELD["qpos"], 0, n, _p(q))
+        total = 0
+        for a in range(ctrl_seq.shape[0]):
+            c = np.ascontiguousarray(ctrl_seq[a], dtype=np.float64)
+            L.orcc_set(h, FIELD["ctrl"], 0, n, _p(c))
+            ops = C.c_ulonglong(0)
+            if L.orcc_step(h, 0, n, substeps, C.byref(ops)) != 0:
+                raise OracleError("mass matrix is not positive definite")
+            if a >= skip:
+                total += ops.value
+        return total / (n * substeps * (ctrl_seq.shape[0] - skip))
+    finally:
+        L.orcc_destroy(h)

@@ -186,6 +186,10 @@ struct ph_sim {
       s_plambda, s_stage, s_gjac, s_grow;
   DBuf<int> s_ncon, s_iters, s_nrows, s_status, s_cgeom, s_pcgeom, s_npcon;
   DBuf<int64_t> d_ids;
+  // k_step timing on the library stream (hipEvents around every launch)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double kernel_ms = 0.0;
+  int64_t kernel_launches = 0;
 };
 
 extern "C" const char* pippin_hip_last_error(void) { return g_err.c_str(); }
@@ -301,6 +305,8 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   s->device = opt ? opt->device : 0;
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&s->ev0));
+  HIPCHK(hipEventCreate(&s->ev1));
 
   DModel& M = s->M;
   phh::init_dmodel_scalars(hm, solver, opt ? opt->max_contacts : 0, M);
@@ -443,6 +449,8 @@ extern "C" void pippin_hip_destroy(ph_sim* s) {
   (void)hipSetDevice(s->device);
   (void)hipStreamSynchronize(s->stream);
   hipStream_t st = s->stream;
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
   delete s;
   if (st) (void)hipStreamDestroy(st);
 }
@@ -482,16 +490,52 @@ static int status_of(ph_sim* s, int64_t b, int64_t e) {
   return PH_OK;
 }
 
+static int field_width(const ph_sim* s, int32_t f, double** dptr) {
+  switch (f) {
+    case PH_FIELD_QPOS: *dptr = s->s_qpos.p; return s->hm.d.nq;
+    case PH_FIELD_QVEL: *dptr = s->s_qvel.p; return s->hm.d.nv;
+    case PH_FIELD_CTRL: *dptr = s->s_ctrl.p; return s->hm.d.nu;
+    default: return -1;
+  }
+}
+
 extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep) {
   if (int rc = check_range(s, begin, end)) return rc;
   if (nstep < 0) return fail(PH_ERR_ARG, "negative step count");
   if (end == begin) return PH_OK;
   HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipEventRecord(s->ev0, s->stream));
   hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin,
                      nstep, (const Real*)nullptr, (Real*)nullptr);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(s->ev1, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  s->kernel_ms += ms;
+  s->kernel_launches++;
   return status_of(s, begin, end);
+}
+
+extern "C" int pippin_hip_kernel_timing(ph_sim* s, int32_t reset, double* total_ms, int64_t* launches) {
+  if (!s) return fail(PH_ERR_ARG, "null sim");
+  if (total_ms) *total_ms = s->kernel_ms;
+  if (launches) *launches = s->kernel_launches;
+  if (reset) {
+    s->kernel_ms = 0.0;
+    s->kernel_launches = 0;
+  }
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_set_device(ph_sim* s, int32_t field, int64_t begin, int64_t end, const double* src) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  double* d;
+  int w = field_width(s, field, &d);
+  if (w < 0 || field == PH_FIELD_QPOS) return fail(PH_ERR_ARG, "set_device takes qvel or ctrl");
+  if (w == 0 || end == begin) return PH_OK;
+  HIPCHK(hipMemcpyAsync(d + begin * w, src, (end - begin) * w * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  return PH_OK;
 }
 
 extern "C" int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const double* ctrl,
@@ -515,14 +559,6 @@ extern "C" int pippin_hip_state_device(ph_sim* s, int32_t field, double** ptr) {
 
 extern "C" void* pippin_hip_stream(ph_sim* s) { return s ? (void*)s->stream : nullptr; }
 
-static int field_width(const ph_sim* s, int32_t f, double** dptr) {
-  switch (f) {
-    case PH_FIELD_QPOS: *dptr = s->s_qpos.p; return s->hm.d.nq;
-    case PH_FIELD_QVEL: *dptr = s->s_qvel.p; return s->hm.d.nv;
-    case PH_FIELD_CTRL: *dptr = s->s_ctrl.p; return s->hm.d.nu;
-    default: return -1;
-  }
-}
 
 extern "C" int pippin_hip_get(const ph_sim* s, int32_t field, int64_t begin, int64_t end, double* out, int64_t len) {
   if (int rc = check_range(s, begin, end)) return rc;

@@ -73,6 +73,8 @@ def lib():
             "pippin_hip_stream": ([vp], vp),
             "pippin_hip_last_error": ([], C.c_char_p),
             "pippin_hip_selftest_math": ([vp, i64, vp, i32], C.c_int),
+            "pippin_hip_set_device": ([vp, i32, i64, i64, vp], C.c_int),
+            "pippin_hip_kernel_timing": ([vp, i32, C.POINTER(C.c_double), C.POINTER(i64)], C.c_int),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
@@ -267,6 +269,15 @@ class HipPhysics:
         p = C.c_void_p()
         self._check(self.L.pippin_hip_state_device(self.h, FIELD[field], C.byref(p)))
         return p.value
+
+    def set_device(self, field, begin, end, dev_ptr: int):
+        """Copy a field from device memory, ordered before the next step."""
+        self._check(self.L.pippin_hip_set_device(self.h, FIELD[field], begin, end, C.c_void_p(dev_ptr)))
+
+    def kernel_timing(self, reset=False):
+        ms, n = C.c_double(0), C.c_int64(0)
+        self._check(self.L.pippin_hip_kernel_timing(self.h, int(reset), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def rollout_device(self, ctrl_ptr: int, qpos_out_ptr: int | None, nstep: int, begin=0, end=None):
         """Batch::rollout with device pointers (e.g. torch.cuda tensors' data_ptr())."""
