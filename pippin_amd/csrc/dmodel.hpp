@@ -22,7 +22,7 @@ enum { ST_NOTPD = 1, ST_CONTACT_OVERFLOW = 2, ST_ROW_OVERFLOW = 4, ST_UNSUPPORTE
 // workgroup's dynamic shared memory, then an int region.
 struct Layout {
   int qpos, qvel, ctrl, qacc, applied;
-  int xpos, xquat, xmat, xipos;
+  int xpos, xquat, xmat, xipos, jq;
   int cdof, cdof_dot, cinert, crb, cvel, cacc, cfrc;
   int qm, chol, H;
   int bias, passive, qfrc_act, actf, actclamp, smooth, qfrc_c, acc, vel;

@@ -384,6 +384,7 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
   L.xquat = take(4 * nb);
   L.xmat = take(9 * nb);
   L.xipos = take(3 * nb);
+  L.jq = take(4 * d.njnt);
   L.cdof = take(6 * nv);
   L.cdof_dot = take(6 * nv);
   L.cinert = take(13 * nb);

@@ -95,7 +95,7 @@ PHD void fk_body(const DModel& M, Env& E, int b) {
       V3 jp = ld3(M.jnt_pos + 3 * j), jax = ld3(M.jnt_axis + 3 * j);
       V3 anchor = pos + qrotate(quat, jp);
       V3 axis = qrotate(quat, jax);
-      quat = qmul(quat, from_axis_angle(jax, q[qa] - M.qpos0[qa]));
+      quat = qmul(quat, ldq(LD(jq) + 4 * j));  // from_axis_angle(axis, q - qpos0), precomputed
       pos = anchor - qrotate(quat, jp);
       stsp(cdof + 6 * da, sp(axis, cross(anchor, axis)));
     } else {
@@ -127,6 +127,14 @@ PHD void fk_body(const DModel& M, Env& E, int b) {
 
 template <class X>
 PHD void kinematics(const DModel& M, Env& E) {
+  // hinge rotations do not depend on the parent's pose: evaluate their
+  // sin/cos for all joints at once, off the tree's serial path
+  X::par(M.njnt, [&](int j) {
+    if (M.jnt_type[j] == J_HINGE) {
+      int qa = M.jnt_qposadr[j];
+      stq(LD(jq) + 4 * j, from_axis_angle(ld3(M.jnt_axis + 3 * j), LD(qpos)[qa] - M.qpos0[qa]));
+    }
+  });
   X::par(1, [&](int) {
     st3(LD(xpos), v3z());
     stq(LD(xquat), q4(1, 0, 0, 0));

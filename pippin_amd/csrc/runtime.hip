@@ -31,8 +31,10 @@ struct DevX {
 // Advance envs [begin, begin + gridDim.x) by nstep steps. With ctrl_seq
 // (envs x nstep x nu) the controls change every step and qpos_out
 // (envs x nstep x nq) receives qpos after each step (Batch::rollout, batch.rs:46-59).
-__global__ void __launch_bounds__(64) k_step(DModel M, DState S, int64_t begin, int64_t nstep, const Real* ctrl_seq,
-                                             Real* qpos_out) {
+__global__ void __launch_bounds__(64) k_step(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                             int64_t begin, int64_t nstep, const Real* ctrl_seq, Real* qpos_out) {
+  const DModel& M = *Mp;
+  const DState& S = *Sp;
   extern __shared__ double lds[];
   Env E;
   E.s = lds;
@@ -43,7 +45,10 @@ __global__ void __launch_bounds__(64) k_step(DModel M, DState S, int64_t begin, 
 }
 
 // Kinematics only (poses after set(qpos) / reset), for listed envs or a range.
-__global__ void __launch_bounds__(64) k_kin(DModel M, DState S, int64_t begin, const int64_t* ids) {
+__global__ void __launch_bounds__(64) k_kin(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                            int64_t begin, const int64_t* ids) {
+  const DModel& M = *Mp;
+  const DState& S = *Sp;
   extern __shared__ double lds[];
   Env E;
   E.s = lds;
@@ -56,7 +61,10 @@ __global__ void __launch_bounds__(64) k_kin(DModel M, DState S, int64_t begin, c
 }
 
 // Data::reset (data.rs:122-131) for listed envs: qacc is kept.
-__global__ void __launch_bounds__(64) k_reset(DModel M, DState S, const int64_t* ids) {
+__global__ void __launch_bounds__(64) k_reset(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                              const int64_t* ids) {
+  const DModel& M = *Mp;
+  const DState& S = *Sp;
   const int64_t e = ids[blockIdx.x];
   const int var = M.env_var[e];
   for (int k = threadIdx.x; k < M.nq; k += 64) S.qpos[e * M.nq + k] = M.v_qpos_reset[(int64_t)var * M.nq + k];
@@ -73,7 +81,10 @@ __global__ void __launch_bounds__(64) k_reset(DModel M, DState S, const int64_t*
 }
 
 // Forward dynamics up to RNE; copies M(q) and c(q, qdot) out.
-__global__ void __launch_bounds__(64) k_mass_bias(DModel M, DState S, int64_t begin, Real* qm, Real* bias) {
+__global__ void __launch_bounds__(64) k_mass_bias(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                                  int64_t begin, Real* qm, Real* bias) {
+  const DModel& M = *Mp;
+  const DState& S = *Sp;
   extern __shared__ double lds[];
   Env E;
   E.s = lds;
@@ -91,7 +102,9 @@ __global__ void __launch_bounds__(64) k_mass_bias(DModel M, DState S, int64_t be
 }
 
 // set_const for listed variants.
-__global__ void __launch_bounds__(64) k_setconst(DModel M, const int* vars, Real* dof_iw, Real* body_iw) {
+__global__ void __launch_bounds__(64) k_setconst(const DModel* __restrict__ Mp, const int* vars, Real* dof_iw,
+                                                 Real* body_iw) {
+  const DModel& M = *Mp;
   extern __shared__ double lds[];
   Env E;
   E.s = lds;
@@ -186,6 +199,8 @@ struct ph_sim {
       s_plambda, s_stage, s_gjac, s_grow;
   DBuf<int> s_ncon, s_iters, s_nrows, s_status, s_cgeom, s_pcgeom, s_npcon;
   DBuf<int64_t> d_ids;
+  DBuf<DModel> d_model;  // device copies of M and S, read through the scalar cache
+  DBuf<DState> d_state;
   // k_step timing on the library stream (hipEvents around every launch)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double kernel_ms = 0.0;
@@ -199,6 +214,15 @@ static hipError_t up(DBuf<T>& b, const std::vector<T>& v, const T** out) {
   hipError_t e = b.upload(v);
   *out = b.p;
   return e;
+}
+
+// Refresh the device copies of the model and state descriptors.
+static int sync_desc(ph_sim* s) {
+  if (!s->d_model.p) HIPCHK(s->d_model.alloc(1));
+  if (!s->d_state.p) HIPCHK(s->d_state.alloc(1));
+  HIPCHK(hipMemcpy(s->d_model.p, &s->M, sizeof(DModel), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->d_state.p, &s->S, sizeof(DState), hipMemcpyHostToDevice));
+  return PH_OK;
 }
 
 static int upload_variants(ph_sim* s) {
@@ -219,7 +243,7 @@ static int upload_variants(ph_sim* s) {
   HIPCHK(up(s->dv_body_invweight, V.body_invweight, &M.v_body_invweight));
   HIPCHK(up(s->dv_qpos_reset, V.qpos_reset, &M.v_qpos_reset));
   HIPCHK(up(s->dv_damped, V.damped, &M.v_damped));
-  return PH_OK;
+  return sync_desc(s);
 }
 
 // Recompute invweight0 on the device for the listed variants and pull it back.
@@ -227,7 +251,7 @@ static int run_setconst(ph_sim* s, const std::vector<int>& vars) {
   if (vars.empty()) return PH_OK;
   DBuf<int> dv;
   HIPCHK(dv.upload(vars));
-  hipLaunchKernelGGL(k_setconst, dim3((unsigned)vars.size()), dim3(64), s->M.L.bytes, s->stream, s->M, dv.p,
+  hipLaunchKernelGGL(k_setconst, dim3((unsigned)vars.size()), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, dv.p,
                      s->dv_dof_invweight.p, s->dv_body_invweight.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -435,8 +459,9 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   if (int rc = run_setconst(s.get(), {0})) return rc;
   if (int rc = upload_variants(s.get())) return rc;
   if (num_envs > 0) {
-    hipLaunchKernelGGL(k_kin, dim3((unsigned)num_envs), dim3(64), M.L.bytes, s->stream, M, S, (int64_t)0,
-                       (const int64_t*)nullptr);
+    if (int rc = sync_desc(s.get())) return rc;
+    hipLaunchKernelGGL(k_kin, dim3((unsigned)num_envs), dim3(64), M.L.bytes, s->stream, s->d_model.p, s->d_state.p,
+                       (int64_t)0, (const int64_t*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
   }
@@ -505,7 +530,7 @@ extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t ns
   if (end == begin) return PH_OK;
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipEventRecord(s->ev0, s->stream));
-  hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin,
+  hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, begin,
                      nstep, (const Real*)nullptr, (Real*)nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, s->stream));
@@ -544,7 +569,7 @@ extern "C" int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, 
   if (nstep < 0 || !ctrl) return fail(PH_ERR_ARG, "bad rollout arguments");
   if (end == begin) return PH_OK;
   HIPCHK(hipSetDevice(s->device));
-  hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin,
+  hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, begin,
                      nstep, ctrl, qpos_out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -580,7 +605,7 @@ extern "C" int pippin_hip_set(ph_sim* s, int32_t field, int64_t begin, int64_t e
   if (w == 0 || end == begin) return PH_OK;
   HIPCHK(hipMemcpy(d + begin * w, src, (end - begin) * w * sizeof(double), hipMemcpyHostToDevice));
   if (field == PH_FIELD_QPOS) {
-    hipLaunchKernelGGL(k_kin, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin,
+    hipLaunchKernelGGL(k_kin, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, begin,
                        (const int64_t*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -595,9 +620,9 @@ extern "C" int pippin_hip_reset(ph_sim* s, const int64_t* envs, int64_t n) {
     if (envs[i] < 0 || envs[i] >= s->n) return fail(PH_ERR_ARG, "env id out of range");
   std::vector<int64_t> ids(envs, envs + n);
   HIPCHK(s->d_ids.upload(ids));
-  hipLaunchKernelGGL(k_reset, dim3((unsigned)n), dim3(64), 0, s->stream, s->M, s->S, s->d_ids.p);
+  hipLaunchKernelGGL(k_reset, dim3((unsigned)n), dim3(64), 0, s->stream, s->d_model.p, s->d_state.p, s->d_ids.p);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_kin, dim3((unsigned)n), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, (int64_t)0, s->d_ids.p);
+  hipLaunchKernelGGL(k_kin, dim3((unsigned)n), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, (int64_t)0, s->d_ids.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   return PH_OK;
@@ -929,7 +954,7 @@ extern "C" int pippin_hip_mass_bias(ph_sim* s, int64_t begin, int64_t end, doubl
   DBuf<double> dq, db;
   HIPCHK(dq.alloc(n * nv * nv));
   HIPCHK(db.alloc(n * nv));
-  hipLaunchKernelGGL(k_mass_bias, dim3((unsigned)n), dim3(64), s->M.L.bytes, s->stream, s->M, s->S, begin, dq.p, db.p);
+  hipLaunchKernelGGL(k_mass_bias, dim3((unsigned)n), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, begin, dq.p, db.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   HIPCHK(hipMemcpy(qm, dq.p, n * nv * nv * 8, hipMemcpyDeviceToHost));
