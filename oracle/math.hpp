@@ -190,6 +190,11 @@ static inline void cr_sincos(double x, double* sout, double* cout) {
     *cout = x - x;
     return;
   }
+  if (x == 0.0) {  // sin(-0) = -0
+    *sout = x;
+    *cout = 1.0;
+    return;
+  }
   const double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
   double k = std::rint(x * 0.6366197723675814);
   DD r = dd_from(x, 0.0);

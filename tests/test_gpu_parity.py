@@ -14,7 +14,7 @@ import pytest
 
 from oracle import mjcf
 from oracle.pyoracle import OracleSim, cr_sincos
-from pippin_amd.physics import HipPhysics, load_model, selftest_math
+from pippin_amd.physics import HipPhysics, selftest_math
 
 pytestmark = pytest.mark.gpu
 
@@ -98,10 +98,3 @@ def test_ant_trajectories_match(assets):
     say("ant 200 steps: worst qpos err", worst, "contacts", int(o.ncon().sum()))
     assert np.array_equal(o.solver_iterations(), g.solver_iterations())
     assert worst == 0.0
-
-
-def test_library_loader_matches_oracle_loader(assets):
-    a = load_model(assets / "ant.xml")
-    b = mjcf.load_file(assets / "ant.xml")
-    for k in b:
-        assert np.array_equal(np.asarray(a[k]).reshape(-1), np.asarray(b[k]).reshape(-1).astype(np.asarray(a[k]).dtype)), k
