@@ -13,6 +13,7 @@
 // the reference's sequence of operations for the values it owns.
 #pragma once
 #include "dcollide.hpp"
+#include "lanes.hpp"
 
 namespace phk {
 
@@ -419,6 +420,11 @@ PHD Real impedance(const Real* solimp, Real pos) {
 // row, each the reference's 4-accumulator dot). Returns false if not PD.
 template <class X>
 PHD bool chol_block(const DModel& M, Env& E, Real* A, int nv, int s0, int e) {
+  if (e - s0 <= GL) {
+    bool ok = chol_lanes<X>(A, nv, s0, e - s0);
+    X::sync();
+    return ok;
+  }
   int* isc = E.si + M.L.i_iscal;
   for (int j = s0; j < e; j++) {
     X::par(1, [&](int) {
@@ -447,6 +453,11 @@ template <class X>
 PHD void chol_solve_block(const DModel& M, Env& E, const Real* L, int nv, int s0, int e, Real* x) {
   (void)M;
   (void)E;
+  if (e - s0 <= GL) {
+    solve_lanes<X>(L, nv, s0, e - s0, x);
+    X::sync();
+    return;
+  }
   X::par(1, [&](int) {
     for (int i = s0; i < e; i++) x[i] = (x[i] - sdot_n(L + i * nv + s0, x + s0, i - s0)) / L[i * nv + i];
     for (int i = e - 1; i >= s0; i--) {

@@ -20,12 +20,13 @@
 using namespace phk;
 
 // ---------------------------------------------------------------- kernels
-struct DevX {
+struct DevX : DevLanes {
   template <class F>
   __device__ __forceinline__ static void par(int n, F&& f) {
     for (int i = (int)threadIdx.x; i < n; i += 64) f(i);
     __syncthreads();
   }
+  __device__ __forceinline__ static void sync() { __syncthreads(); }
 };
 
 // Advance envs [begin, begin + gridDim.x) by nstep steps. With ctrl_seq

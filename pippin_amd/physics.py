@@ -43,6 +43,12 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise HipError(f"{LIB_PATH} is missing: build it with `python -m pippin_amd.build`")
+        # One HIP runtime per process: torch ships its own libamdhip64 under the
+        # same soname. Loading torch first binds this library to that copy, so
+        # torch tensors and this library share the device; loading it second
+        # would start a second runtime that finds no GPU.
+        import torch  # noqa: F401
+
         L = C.CDLL(str(LIB_PATH))
         vp, i64, i32 = C.c_void_p, C.c_int64, C.c_int32
         sigs = {

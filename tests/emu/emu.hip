@@ -17,11 +17,12 @@
 
 using namespace phk;
 
-struct HostX {
+struct HostX : HostLanes {
   template <class F>
   static void par(int n, F&& f) {
     for (int i = 0; i < n; i++) f(i);
   }
+  static void sync() {}
 };
 
 struct emu_sim {

@@ -18,6 +18,9 @@ def lib():
     global _lib
     if _lib is None:
         subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+        # the emulation links libamdhip64; bind it to torch's copy (one HIP runtime per process)
+        import torch  # noqa: F401
+
         L = C.CDLL(str(LIB))
         vp, i64 = C.c_void_p, C.c_int64
         L.emu_create.restype = vp
