@@ -364,7 +364,14 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
 
 }
 
-// LDS layout of one env (offsets in doubles, then ints).
+// LDS layout of one env (offsets in doubles, then ints). Arrays live across
+// the whole step are laid out first; three phase regions then share one
+// span, since their live ranges within a step are disjoint:
+//   A: crba/velocity/rne temporaries (composite inertias, cvel, cdof_dot, cacc, cfrc)
+//   B: collision (world AABBs)
+//   C: constraint rows and Newton vectors (built after collision, dead after the solve)
+// The Newton Hessian reuses the mass factor's storage: the factor of M is
+// consumed by the unconstrained solve before Newton and rebuilt after it.
 phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax) {
   phk::Layout L;
   const ph_model_desc& d = hm.d;
@@ -386,15 +393,10 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
   L.xipos = take(3 * nb);
   L.jq = take(4 * d.njnt);
   L.cdof = take(6 * nv);
-  L.cdof_dot = take(6 * nv);
   L.cinert = take(13 * nb);
-  L.crb = take(13 * nb);
-  L.cvel = take(6 * nb);
-  L.cacc = take(6 * nb);
-  L.cfrc = take(6 * nb);
   L.qm = take(nv * nv);
   L.chol = take(nv * nv);
-  L.H = take(nv * nv);
+  L.H = L.chol;
   L.bias = take(nv);
   L.passive = take(nv);
   L.qfrc_act = take(nv);
@@ -406,7 +408,21 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
   L.vel = take(nv);
   L.gxpos = take(3 * ng);
   L.gxmat = take(9 * ng);
+  L.scal = take(16);
+  const int region = o;
+  // A
+  L.crb = take(13 * nb);
+  L.cvel = take(6 * nb);
+  L.cdof_dot = take(6 * nv);
+  L.cacc = take(6 * nb);
+  L.cfrc = take(6 * nb);
+  int end = o;
+  // B
+  o = region;
   L.aabb = take(6 * ng);
+  end = std::max(end, o);
+  // C
+  o = region;
   L.a = take(nv);
   L.a0 = take(nv);
   L.grad = take(nv);
@@ -424,7 +440,7 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
   L.ract = take(nrl);
   L.rprev = take(nrl);
   L.rf = take(nrl);
-  L.scal = take(16);
+  o = std::max(end, o);
   L.ndouble = o;
   int oi = 0;
   L.i_paircnt = oi;
@@ -436,7 +452,8 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
   L.i_iscal = oi;
   oi += 16;
   L.nint = oi;
-  L.bytes = (int)(L.ndouble * sizeof(double) + L.nint * sizeof(int));
+  // 16-byte aligned so that envs can be packed side by side in one workgroup
+  L.bytes = (int)((L.ndouble * sizeof(double) + L.nint * sizeof(int) + 15) / 16 * 16);
   return L;
 }
 

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -20,29 +21,41 @@
 using namespace phk;
 
 // ---------------------------------------------------------------- kernels
-struct DevX : DevLanes {
+// K environments per workgroup of one 64-lane wavefront: env g of the
+// workgroup owns lanes [g * 64/K, (g + 1) * 64/K) and its own slice of the LDS.
+// Loops over an env's items stride over its lanes; the dense linear algebra
+// uses the first 16 of them (one DPP row).
+template <int K>
+struct DevXK : DevLanes {
+  static constexpr int G = 64 / K;
   template <class F>
   __device__ __forceinline__ static void par(int n, F&& f) {
-    for (int i = (int)threadIdx.x; i < n; i += 64) f(i);
+    for (int i = (int)(threadIdx.x % G); i < n; i += G) f(i);
     __syncthreads();
   }
   __device__ __forceinline__ static void sync() { __syncthreads(); }
 };
+using DevX = DevXK<1>;
 
-// Advance envs [begin, begin + gridDim.x) by nstep steps. With ctrl_seq
+// Advance envs [begin, end) by nstep steps, K per workgroup. With ctrl_seq
 // (envs x nstep x nu) the controls change every step and qpos_out
 // (envs x nstep x nq) receives qpos after each step (Batch::rollout, batch.rs:46-59).
+template <int K>
 __global__ void __launch_bounds__(64) k_step(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
-                                             int64_t begin, int64_t nstep, const Real* ctrl_seq, Real* qpos_out) {
+                                             int64_t begin, int64_t end, int64_t nstep, const Real* ctrl_seq,
+                                             Real* qpos_out) {
   const DModel& M = *Mp;
   const DState& S = *Sp;
   extern __shared__ double lds[];
+  const int g = (int)threadIdx.x / DevXK<K>::G;
+  const int64_t rel = (int64_t)blockIdx.x * K + g;
+  if (begin + rel >= end) return;  // tail of the last workgroup
   Env E;
-  E.s = lds;
-  E.si = reinterpret_cast<int*>(lds + M.L.ndouble);
-  E.e = begin + blockIdx.x;
+  E.s = lds + (int64_t)g * (M.L.bytes / 8);
+  E.si = reinterpret_cast<int*>(E.s + M.L.ndouble);
+  E.e = begin + rel;
   E.var = M.env_var[E.e];
-  run_env<DevX>(M, S, E, nstep, ctrl_seq, qpos_out, blockIdx.x);
+  run_env<DevXK<K>>(M, S, E, nstep, ctrl_seq, qpos_out, rel);
 }
 
 // Kinematics only (poses after set(qpos) / reset), for listed envs or a range.
@@ -178,6 +191,7 @@ struct ph_sim {
   phh::HostModel hm;
   int64_t n = 0;
   int device = 0;
+  int envs_per_wg = 1;  // environments packed into one k_step workgroup
   hipStream_t stream = nullptr;
   DModel M;
   DState S;
@@ -336,6 +350,17 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   DModel& M = s->M;
   phh::init_dmodel_scalars(hm, solver, opt ? opt->max_contacts : 0, M);
   if (M.L.bytes > 64 * 1024) return fail(PH_ERR_MODEL, "model too large for the per-env LDS working set");
+  // Pack as many envs per workgroup as fit 64 KiB of LDS (PIPPIN_HIP_ENVS_PER_WG overrides: 1, 2 or 4).
+  s->envs_per_wg = 1;
+  for (int k : {4, 2})
+    if (k * M.L.bytes <= 64 * 1024) {
+      s->envs_per_wg = k;
+      break;
+    }
+  if (const char* ev = std::getenv("PIPPIN_HIP_ENVS_PER_WG")) {
+    int k = std::atoi(ev);
+    if ((k == 1 || k == 2 || k == 4) && k * M.L.bytes <= 64 * 1024) s->envs_per_wg = k;
+  }
 
   int ii = 0, fi = 0;
   auto ui = [&](const std::vector<int32_t>& v, const int** dst) -> hipError_t { return up(s->d_i[ii++], v, dst); };
@@ -525,14 +550,29 @@ static int field_width(const ph_sim* s, int32_t f, double** dptr) {
   }
 }
 
+// One k_step launch over [begin, end) with the env packing chosen at create.
+static void launch_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const Real* ctrl, Real* qpos_out) {
+  const int K = s->envs_per_wg;
+  const unsigned grid = (unsigned)((end - begin + K - 1) / K);
+  const size_t lds = (size_t)K * s->M.L.bytes;
+  if (K == 4)
+    hipLaunchKernelGGL(k_step<4>, dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep,
+                       ctrl, qpos_out);
+  else if (K == 2)
+    hipLaunchKernelGGL(k_step<2>, dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep,
+                       ctrl, qpos_out);
+  else
+    hipLaunchKernelGGL(k_step<1>, dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep,
+                       ctrl, qpos_out);
+}
+
 extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep) {
   if (int rc = check_range(s, begin, end)) return rc;
   if (nstep < 0) return fail(PH_ERR_ARG, "negative step count");
   if (end == begin) return PH_OK;
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipEventRecord(s->ev0, s->stream));
-  hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, begin,
-                     nstep, (const Real*)nullptr, (Real*)nullptr);
+  launch_step(s, begin, end, nstep, nullptr, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -570,8 +610,7 @@ extern "C" int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, 
   if (nstep < 0 || !ctrl) return fail(PH_ERR_ARG, "bad rollout arguments");
   if (end == begin) return PH_OK;
   HIPCHK(hipSetDevice(s->device));
-  hipLaunchKernelGGL(k_step, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, begin,
-                     nstep, ctrl, qpos_out);
+  launch_step(s, begin, end, nstep, ctrl, qpos_out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   return status_of(s, begin, end);
