@@ -23,18 +23,19 @@ enum { ST_NOTPD = 1, ST_CONTACT_OVERFLOW = 2, ST_ROW_OVERFLOW = 4, ST_UNSUPPORTE
 struct Layout {
   int qpos, qvel, ctrl, qacc, applied;
   int xpos, xquat, xmat, xipos, jq;
-  int cdof, cdof_dot, cinert, crb, cvel, cacc, cfrc;
+  int cdof, cinert, crb, cvel, cacc, cfrc;
   int qm, chol, H;
   int bias, passive, qfrc_act, actf, actclamp, smooth, qfrc_c, acc, vel;
   int gxpos, gxmat, aabb;
+  int paircnt, pairoff, cact;  // int arrays stored in double slots
   int a, a0, grad, p, tmp, tmp2, mp;
   int J, aref, dd, jar, jp, rpos, rdiag, ract, rprev, rf;
-  int scal;     // 16 doubles of wave-uniform scalars
+  int scal;     // 16 doubles of per-env scalars
   int ndouble;  // total doubles
   // int region (offsets in ints, after the doubles)
-  int i_paircnt, i_pairoff, i_cact, i_iscal;
+  int i_iscal;
   int nint;
-  int bytes;
+  int bytes;  // per env, 16-byte aligned
 };
 
 struct DModel {

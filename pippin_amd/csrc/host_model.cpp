@@ -358,20 +358,34 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
   M.ncmax = std::max(hm.ncmax, 1);
   if (max_contacts > 0) M.ncmax = std::min(M.ncmax, max_contacts);
   M.nrmax = std::max(4 * M.ncmax + 2 * hm.nlimited + d.neq, 1);
-  // keep the LDS footprint modest: rows beyond nrl live in HBM
+  // Rows beyond nrl live in HBM. Keep up to 32 in LDS, fewer if that keeps
+  // the per-env working set within 10 KiB (four 4-env workgroups per CU).
   M.nrl = std::min(M.nrmax, 32);
   M.L = make_layout(hm, M.nrl, M.npair, M.ncmax);
+  while (M.L.bytes > 10 * 1024 && M.nrl > 16) {
+    M.nrl -= 4;
+    M.L = make_layout(hm, M.nrl, M.npair, M.ncmax);
+  }
 
 }
 
-// LDS layout of one env (offsets in doubles, then ints). Arrays live across
-// the whole step are laid out first; three phase regions then share one
-// span, since their live ranges within a step are disjoint:
-//   A: crba/velocity/rne temporaries (composite inertias, cvel, cdof_dot, cacc, cfrc)
-//   B: collision (world AABBs)
-//   C: constraint rows and Newton vectors (built after collision, dead after the solve)
-// The Newton Hessian reuses the mass factor's storage: the factor of M is
-// consumed by the unconstrained solve before Newton and rebuilt after it.
+// LDS layout of one env (offsets in doubles, then ints).
+//
+// Arrays live across a whole step come first. The rest share one span, by
+// live range within a step (kinematics -> crba -> velocity/rne -> collision ->
+// rows/Newton/integration):
+//   S1 (kinematics .. collision): cinert, xpos, xmat
+//   after S1, mutually exclusive in time:
+//     kinematics temporaries: xquat, hinge quaternions
+//     crba: composite inertias
+//     velocity/rne: cvel, cacc, cfrc
+//     collision: geom poses, world AABBs, per-pair hit counts and offsets
+//   C (after collision, over S1 and all of the above): Newton vectors, the
+//     mass factor (which the Newton Hessian reuses: the factor of M is consumed
+//     by the unconstrained solve before Newton and rebuilt after it), the
+//     touching-contact list, constraint rows (row positions and diagonals share
+//     storage with J*a and J*p, which Newton fills only after the rows are built).
+// Poses for export are recomputed by the final kinematics pass of a launch.
 phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax) {
   phk::Layout L;
   const ph_model_desc& d = hm.d;
@@ -382,21 +396,15 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
     o += n;
     return r;
   };
+  auto ints = [](int n) { return (n + 1) / 2; };  // int32 arrays in double slots
   L.qpos = take(nq);
   L.qvel = take(nv);
   L.ctrl = take(nu);
   L.qacc = take(nv);
   L.applied = take(nv);
-  L.xpos = take(3 * nb);
-  L.xquat = take(4 * nb);
-  L.xmat = take(9 * nb);
   L.xipos = take(3 * nb);
-  L.jq = take(4 * d.njnt);
   L.cdof = take(6 * nv);
-  L.cinert = take(13 * nb);
   L.qm = take(nv * nv);
-  L.chol = take(nv * nv);
-  L.H = L.chol;
   L.bias = take(nv);
   L.passive = take(nv);
   L.qfrc_act = take(nv);
@@ -406,23 +414,33 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
   L.qfrc_c = take(nv);
   L.acc = take(nv);
   L.vel = take(nv);
-  L.gxpos = take(3 * ng);
-  L.gxmat = take(9 * ng);
   L.scal = take(16);
-  const int region = o;
-  // A
+  const int span = o;
+  L.cinert = take(13 * nb);
+  L.xpos = take(3 * nb);
+  L.xmat = take(9 * nb);
+  const int r1 = o;
+  int end = o;
+  o = r1;  // kinematics temporaries
+  L.xquat = take(4 * nb);
+  L.jq = take(4 * d.njnt);
+  end = std::max(end, o);
+  o = r1;  // crba
   L.crb = take(13 * nb);
+  end = std::max(end, o);
+  o = r1;  // velocity / rne
   L.cvel = take(6 * nb);
-  L.cdof_dot = take(6 * nv);
   L.cacc = take(6 * nb);
   L.cfrc = take(6 * nb);
-  int end = o;
-  // B
-  o = region;
-  L.aabb = take(6 * ng);
   end = std::max(end, o);
-  // C
-  o = region;
+  o = r1;  // collision
+  L.gxpos = take(3 * ng);
+  L.gxmat = take(9 * ng);
+  L.aabb = take(6 * ng);
+  L.paircnt = take(ints(npair));
+  L.pairoff = take(ints(npair));
+  end = std::max(end, o);
+  o = span;  // C
   L.a = take(nv);
   L.a0 = take(nv);
   L.grad = take(nv);
@@ -430,25 +448,22 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
   L.tmp = take(nv);
   L.tmp2 = take(nv);
   L.mp = take(nv);
+  L.chol = take(nv * nv);
+  L.H = L.chol;
+  L.cact = take(ints(ncmax));
   L.J = take(nrl * nv);
   L.aref = take(nrl);
   L.dd = take(nrl);
   L.jar = take(nrl);
   L.jp = take(nrl);
-  L.rpos = take(nrl);
-  L.rdiag = take(nrl);
+  L.rpos = L.jar;
+  L.rdiag = L.jp;
   L.ract = take(nrl);
   L.rprev = take(nrl);
   L.rf = take(nrl);
   o = std::max(end, o);
   L.ndouble = o;
   int oi = 0;
-  L.i_paircnt = oi;
-  oi += npair;
-  L.i_pairoff = oi;
-  oi += npair;
-  L.i_cact = oi;
-  oi += ncmax;
   L.i_iscal = oi;
   oi += 16;
   L.nint = oi;

@@ -148,6 +148,13 @@ PHD void kinematics(const DModel& M, Env& E) {
     int b0 = M.level_adr[l], nb = M.level_adr[l + 1] - b0;
     X::par(nb, [&](int i) { fk_body<X>(M, E, M.level_body[b0 + i]); });
   }
+}
+
+// Geom world poses, the last loop of forward.rs:14-90. Evaluated where they
+// are consumed (collision, pose export) so that they need no LDS of their own
+// across the dynamics stages.
+template <class X>
+PHD void geom_poses(const DModel& M, Env& E) {
   X::par(M.ngeom, [&](int g) {
     int b = M.geom_body[g];
     M3 xm = ldm(LD(xmat) + 9 * b);
@@ -187,50 +194,23 @@ PHD void crba(const DModel& M, Env& E) {
   });
 }
 
-// ---------------- velocity, forward.rs:115-144 ----------------
+// ---------------- velocity + RNE, forward.rs:115-171 ----------------
+// One pass over the tree levels computes, per body, cvel and cdof_dot
+// (forward.rs:115-144) and immediately the RNE forward terms cacc and cfrc
+// (forward.rs:147-165): cdof_dot of a dof is consumed by its own body's
+// acceleration sum in dof order, exactly where the reference's second loop
+// would read it, so it never needs storage. Then the backward force sum and
+// qfrc_bias (forward.rs:166-171).
 template <class X>
-PHD void velocity(const DModel& M, Env& E) {
+PHD void velocity_rne(const DModel& M, Env& E) {
   Real* cvel = LD(cvel);
-  Real* cdd = LD(cdof_dot);
-  const Real* cd = LD(cdof);
-  const Real* qvel = LD(qvel);
-  X::par(1, [&](int) { stsp(cvel, sp(v3z(), v3z())); });
-  for (int l = 0; l < M.nlevel; l++) {
-    int b0 = M.level_adr[l], nb = M.level_adr[l + 1] - b0;
-    X::par(nb, [&](int i) {
-      int b = M.level_body[b0 + i];
-      SP v = ldsp(cvel + 6 * M.body_parent[b]);
-      int j0 = M.body_jntadr[b], j1 = j0 + M.body_jntnum[b];
-      for (int j = j0; j < j1; j++) {
-        int da = M.jnt_dofadr[j], t = M.jnt_type[j];
-        if (t == J_FREE || t == J_BALL) {
-          int lin = t == J_FREE ? 3 : 0, rot = t == J_FREE ? da + 3 : da;
-          for (int k = 0; k < lin; k++) {
-            stsp(cdd + 6 * (da + k), sp(v3z(), v3z()));
-            v = spadd(v, sscale(ldsp(cd + 6 * (da + k)), qvel[da + k]));
-          }
-          for (int k = 0; k < 3; k++) stsp(cdd + 6 * (rot + k), cross_motion(v, ldsp(cd + 6 * (rot + k))));
-          for (int k = 0; k < 3; k++) v = spadd(v, sscale(ldsp(cd + 6 * (rot + k)), qvel[rot + k]));
-        } else {
-          stsp(cdd + 6 * da, cross_motion(v, ldsp(cd + 6 * da)));
-          v = spadd(v, sscale(ldsp(cd + 6 * da), qvel[da]));
-        }
-      }
-      stsp(cvel + 6 * b, v);
-    });
-  }
-}
-
-// ---------------- RNE, forward.rs:147-171 ----------------
-template <class X>
-PHD void rne(const DModel& M, Env& E) {
   Real* cacc = LD(cacc);
   Real* cfrc = LD(cfrc);
-  const Real* cvel = LD(cvel);
-  const Real* cdd = LD(cdof_dot);
+  const Real* cd = LD(cdof);
   const Real* ci = LD(cinert);
   const Real* qvel = LD(qvel);
   X::par(1, [&](int) {
+    stsp(cvel, sp(v3z(), v3z()));
     stsp(cacc, sp(v3z(), -v3(M.gravity[0], M.gravity[1], M.gravity[2])));
     stsp(cfrc, sp(v3z(), v3z()));
   });
@@ -238,11 +218,27 @@ PHD void rne(const DModel& M, Env& E) {
     int b0 = M.level_adr[l], nb = M.level_adr[l + 1] - b0;
     X::par(nb, [&](int i) {
       int b = M.level_body[b0 + i];
-      SP a = ldsp(cacc + 6 * M.body_parent[b]);
-      int k0 = M.body_dofadr[b], k1 = k0 + M.body_dofnum[b];
-      for (int k = k0; k < k1; k++) a = spadd(a, sscale(ldsp(cdd + 6 * k), qvel[k]));
+      int par = M.body_parent[b];
+      SP v = ldsp(cvel + 6 * par);
+      SP a = ldsp(cacc + 6 * par);
+      int j0 = M.body_jntadr[b], j1 = j0 + M.body_jntnum[b];
+      for (int j = j0; j < j1; j++) {
+        int da = M.jnt_dofadr[j], t = M.jnt_type[j];
+        if (t == J_FREE || t == J_BALL) {
+          int lin = t == J_FREE ? 3 : 0, rot = t == J_FREE ? da + 3 : da;
+          for (int k = 0; k < lin; k++) {
+            a = spadd(a, sscale(sp(v3z(), v3z()), qvel[da + k]));
+            v = spadd(v, sscale(ldsp(cd + 6 * (da + k)), qvel[da + k]));
+          }
+          for (int k = 0; k < 3; k++) a = spadd(a, sscale(cross_motion(v, ldsp(cd + 6 * (rot + k))), qvel[rot + k]));
+          for (int k = 0; k < 3; k++) v = spadd(v, sscale(ldsp(cd + 6 * (rot + k)), qvel[rot + k]));
+        } else {
+          a = spadd(a, sscale(cross_motion(v, ldsp(cd + 6 * da)), qvel[da]));
+          v = spadd(v, sscale(ldsp(cd + 6 * da), qvel[da]));
+        }
+      }
+      stsp(cvel + 6 * b, v);
       stsp(cacc + 6 * b, a);
-      SP v = ldsp(cvel + 6 * b);
       SP h = mul_motion(ci + 13 * b, v);
       stsp(cfrc + 6 * b, spadd(mul_motion(ci + 13 * b, a), cross_force(v, h)));
     });
@@ -306,6 +302,7 @@ PHD void passive_actuation(const DModel& M, Env& E) {
 template <class X>
 PHD void collide(const DModel& M, const DState& S, Env& E) {
   const int ng = M.ngeom;
+  geom_poses<X>(M, E);
   Real* aabb = LD(aabb);
   const Real margin = M.margin;
   X::par(ng, [&](int g) {
@@ -323,7 +320,7 @@ PHD void collide(const DModel& M, const DState& S, Env& E) {
         v3(fabs(r[0]) * h.x + fabs(r[1]) * h.y + fabs(r[2]) * h.z, fabs(r[3]) * h.x + fabs(r[4]) * h.y + fabs(r[5]) * h.z,
            fabs(r[6]) * h.x + fabs(r[7]) * h.y + fabs(r[8]) * h.z));
   });
-  int* cnt = E.si + M.L.i_paircnt;
+  int* cnt = reinterpret_cast<int*>(LD(paircnt));
   Real* stage = S.stage + E.e * (int64_t)M.npair * MAXHIT * CW;
   X::par(M.npair, [&](int p) {
     int g1 = M.pair_g1[p], g2 = M.pair_g2[p];
@@ -351,7 +348,7 @@ PHD void collide(const DModel& M, const DState& S, Env& E) {
     cnt[p] = sink.n;
   });
   // pair-ordered compaction (contacts are listed in pair order, then hit order)
-  int* off = E.si + M.L.i_pairoff;
+  int* off = reinterpret_cast<int*>(LD(pairoff));
   int* isc = E.si + M.L.i_iscal;
   X::par(1, [&](int) {
     int o = 0;
@@ -509,7 +506,7 @@ template <class X>
 PHD int build_rows(const DModel& M, const DState& S, Env& E) {
   const int nv = M.nv;
   int* isc = E.si + M.L.i_iscal;
-  int* cact = E.si + M.L.i_cact;
+  int* cact = reinterpret_cast<int*>(LD(cact));
   const Real* con = S.con + E.e * (int64_t)M.ncmax * CW;
   const int* cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
   const Real* qpos = LD(qpos);
@@ -839,8 +836,7 @@ PHD bool step_env(const DModel& M, const DState& S, Env& E) {
   int* isc = E.si + M.L.i_iscal;
   kinematics<X>(M, E);
   crba<X>(M, E);
-  velocity<X>(M, E);
-  rne<X>(M, E);
+  velocity_rne<X>(M, E);
   passive_actuation<X>(M, E);
   collide<X>(M, S, E);
   if (isc[7] & ST_CONTACT_OVERFLOW) return false;
@@ -998,6 +994,7 @@ PHD void load_state(const DModel& M, const DState& S, Env& E) {
 
 template <class X>
 PHD void store_poses(const DModel& M, const DState& S, Env& E) {
+  geom_poses<X>(M, E);
   const Real* s = E.s;
   const int64_t e = E.e;
   X::par(3 * M.ngeom, [&](int k) { S.geom_xpos[e * 3 * M.ngeom + k] = s[M.L.gxpos + k]; });

@@ -108,8 +108,7 @@ __global__ void __launch_bounds__(64) k_mass_bias(const DModel* __restrict__ Mp,
   load_state<DevX>(M, S, E);
   kinematics<DevX>(M, E);
   crba<DevX>(M, E);
-  velocity<DevX>(M, E);
-  rne<DevX>(M, E);
+  velocity_rne<DevX>(M, E);
   const int64_t rel = blockIdx.x;
   DevX::par(M.nv * M.nv, [&](int k) { qm[rel * M.nv * M.nv + k] = lds[M.L.qm + k]; });
   DevX::par(M.nv, [&](int k) { bias[rel * M.nv + k] = lds[M.L.bias + k]; });
@@ -350,7 +349,11 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   DModel& M = s->M;
   phh::init_dmodel_scalars(hm, solver, opt ? opt->max_contacts : 0, M);
   if (M.L.bytes > 64 * 1024) return fail(PH_ERR_MODEL, "model too large for the per-env LDS working set");
-  // Pack as many envs per workgroup as fit 64 KiB of LDS (PIPPIN_HIP_ENVS_PER_WG overrides: 1, 2 or 4).
+  // Pack as many envs per workgroup as fit 64 KiB of LDS (PIPPIN_HIP_ENVS_PER_WG overrides: 1, 2 or 4,
+  // up to the device's per-workgroup LDS).
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, s->device));
+  const int64_t lds_max = (int64_t)prop.sharedMemPerBlock;
   s->envs_per_wg = 1;
   for (int k : {4, 2})
     if (k * M.L.bytes <= 64 * 1024) {
@@ -359,7 +362,13 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
     }
   if (const char* ev = std::getenv("PIPPIN_HIP_ENVS_PER_WG")) {
     int k = std::atoi(ev);
-    if ((k == 1 || k == 2 || k == 4) && k * M.L.bytes <= 64 * 1024) s->envs_per_wg = k;
+    if ((k == 1 || k == 2 || k == 4) && (int64_t)k * M.L.bytes <= lds_max) s->envs_per_wg = k;
+  }
+  {
+    const int k = s->envs_per_wg;
+    const int bytes = k * M.L.bytes;
+    const void* fn = k == 4 ? (const void*)k_step<4> : k == 2 ? (const void*)k_step<2> : (const void*)k_step<1>;
+    if (bytes > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   }
 
   int ii = 0, fi = 0;

@@ -288,8 +288,7 @@ int emu_mass_bias(emu_sim* s, int64_t begin, int64_t end, double* qm, double* bi
     load_state<HostX>(M, s->S, E);
     kinematics<HostX>(M, E);
     crba<HostX>(M, E);
-    velocity<HostX>(M, E);
-    rne<HostX>(M, E);
+    velocity_rne<HostX>(M, E);
     memcpy(qm + (e - begin) * M.nv * M.nv, E.s + M.L.qm, M.nv * M.nv * sizeof(double));
     memcpy(bias + (e - begin) * M.nv, E.s + M.L.bias, M.nv * sizeof(double));
   }

@@ -13,6 +13,7 @@ import pytest
 from oracle import mjcf
 from oracle.pyoracle import OracleSim
 from tests.emu.pyemu import EmuSim
+from tests.scenes import PRIMITIVE_SCENES, primitive_xml
 
 
 def _perturbed(q, rng, scale, free=True):
@@ -88,3 +89,29 @@ def test_contact_free_models_exact(assets, name, free):
             s.step(0, n, 1)
     assert np.array_equal(o.get("qpos"), e.get("qpos"))
     assert np.array_equal(o.get("qvel"), e.get("qvel"))
+
+
+@pytest.mark.parametrize("name", sorted(PRIMITIVE_SCENES))
+def test_primitive_pairs_exact(name):
+    arr = mjcf.load_string(primitive_xml(name))
+    n = 6
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(11)
+    q = o.get("qpos")
+    q[:, :3] += rng.normal(size=(n, 3)) * 0.02
+    v = rng.normal(size=(n, 6)) * 0.3
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+        s.set("qvel", 0, n, v)
+    touched = 0
+    for t in range(250):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        touched += int((o.ncon() > 0).sum())
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert touched > 0, "the scene never made contact"
+    for k in range(n):
+        co, ce = o.contacts(k), e.contacts(k)
+        for f in ("geom1", "geom2", "feature", "depth", "pos", "normal", "tangent"):
+            assert np.array_equal(co[f], ce[f]), f

@@ -15,6 +15,7 @@ import pytest
 from oracle import mjcf
 from oracle.pyoracle import OracleSim, cr_sincos
 from pippin_amd.physics import HipPhysics, selftest_math
+from tests.scenes import PRIMITIVE_SCENES, primitive_xml
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +99,30 @@ def test_ant_trajectories_match(assets):
     say("ant 200 steps: worst qpos err", worst, "contacts", int(o.ncon().sum()))
     assert np.array_equal(o.solver_iterations(), g.solver_iterations())
     assert worst == 0.0
+
+
+@pytest.mark.parametrize("name", sorted(PRIMITIVE_SCENES))
+def test_primitive_pairs_match(name):
+    """Every primitive pair of the device narrow phase, 250 steps through contact."""
+    arr = mjcf.load_string(primitive_xml(name))
+    n = 64
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(11)
+    q = o.get("qpos")
+    q[:, :3] += rng.normal(size=(n, 3)) * 0.02
+    v = rng.normal(size=(n, 6)) * 0.3
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+        s.set("qvel", 0, n, v)
+    say("primitive pair", name)
+    for t in range(250):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        err = np.abs(o.get("qpos") - g.get("qpos")).max()
+        assert err <= 1e-9, (t, err)
+    assert np.array_equal(o.get("qpos"), g.get("qpos"))
+    for k in range(0, n, 9):
+        co, cg = o.contacts(k), g.contacts(k)
+        for f in ("geom1", "geom2", "feature", "depth", "pos", "normal", "tangent"):
+            assert np.array_equal(co[f], cg[f]), f

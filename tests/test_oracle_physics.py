@@ -262,3 +262,19 @@ def test_ballistic_free_body():
     z = 5 + h * (n * 2.0 - 9.81 * h * n * (n + 1) / 2)
     assert np.allclose(q[:3], [h * n * 1.0, -h * n * 0.5, z], atol=1e-12)
     assert np.allclose(q[3:7], [1, 0, 0, 0], atol=0)
+
+
+# plane-cylinder (collision.rs plane_cylinder) has no reference test: pin it on
+# rest heights, upright (half-height) and lying on its side (radius).
+@both
+def test_cylinder_rests_upright_and_on_its_side(solver):
+    s = sim_xml("""<mujoco><option timestep="0.002"/><worldbody>
+        <geom type="plane" size="5 5 .1"/>
+        <body pos="0 0 0.3"><freejoint/><geom type="cylinder" size=".05 .08"/></body>
+        <body pos="1 0 0.3" euler="90 0 0"><freejoint/><geom type="cylinder" size=".04 .1"/></body>
+    </worldbody></mujoco>""", solver=solver)
+    run(s, 3.0, 0.002)
+    q = s.get("qpos")[0]
+    assert abs(q[2] - 0.08) < 2e-3, q[2]
+    assert abs(q[9] - 0.04) < 2e-3, q[9]
+    assert np.linalg.norm(s.get("qvel")[0]) < 1e-2
