@@ -647,8 +647,171 @@ PHD void mul_m(const DModel& M, Env& E, const Real* x, Real* y) {
   X::par(nv, [&](int i) { y[i] = sdot_n(qm + i * nv, x, nv) + 0.0 * x[i]; });
 }
 
+// newton.rs:224-371 for nv <= 16 and nr <= 16 rows held in LDS, with the
+// vectors in registers: a dof vector holds dof k in lane k, a row vector holds
+// row r in lane r, exchanged by broadcasts inside the 16-lane row. M and J stay
+// in LDS and are read with independent loads; the Hessian is assembled one row
+// per lane directly into the register Cholesky. Every sum keeps the
+// reference's order, so the result equals newton_dense's bit for bit.
+template <class X>
+PHD int newton_lanes(const DModel& M, Env& E, int nr) {
+  using LV = typename X::LV;
+  const int nv = M.nv;
+  const int n_ineq = nr - M.neq;
+  const Real* qm = LD(qm);
+  const Real* J = E.J;
+  Real* Lf = LD(chol);
+  const LV aref = X::gen([&](int l) { return l < nr ? E.aref[l] : 0.0; });
+  const LV dd = X::gen([&](int l) { return l < nr ? E.dd[l] : 0.0; });
+  LV ddb[GL];
+  bcast_all<X>(dd, ddb);
+  // active(v): bilateral rows always, inequality rows when v < 0
+  auto active_mask = [&](const LV& v) {
+    return X::ballot([&](int l) { return l < nr && (l >= n_ineq || X::at(v, l) < 0.0); });
+  };
+  // M x (damp_h is zero inside Newton: damping is applied after the solve)
+  auto mul_m = [&](const LV& x) {
+    LV xb[GL], mr[GL];
+    bcast_all<X>(x, xb);
+#pragma unroll
+    for (int k = 0; k < GL; k++) mr[k] = X::gen([&](int l) { return (l < nv && k < nv) ? qm[l * nv + k] : 0.0; });
+    return lsdot_rt<X>(mr, xb, nv) + 0.0 * x;
+  };
+  // J x, one row per lane
+  auto jdot = [&](const LV& x) {
+    LV xb[GL], jr[GL];
+    bcast_all<X>(x, xb);
+#pragma unroll
+    for (int k = 0; k < GL; k++) jr[k] = X::gen([&](int l) { return (l < nr && k < nv) ? J[l * nv + k] : 0.0; });
+    return lsdot_rt<X>(jr, xb, nv);
+  };
+  // column r of J^T: lane k gets J[r][k]
+  auto jrow = [&](int r) { return X::gen([&](int l) { return l < nv ? J[r * nv + l] : 0.0; }); };
+  // sum over active rows of f_r J_r (a dof vector), rows in order, from `init`
+  auto add_rows = [&](LV g, const LV& f, uint32_t act) {
+    LV fb[GL];
+    bcast_all<X>(f, fb);
+#pragma unroll
+    for (int r = 0; r < GL; r++)
+      if (r < nr && ((act >> r) & 1u)) g = g + fb[r] * jrow(r);
+    return g;
+  };
+  auto cost = [&](const LV& x, const LV& a0v) {
+    LV t = x - a0v;
+    LV t2 = mul_m(t);
+    Real c = 0.5 * X::uni(lsum_lanes<X>(t * t2, nv, -0.0));
+    LV jar = jdot(x) - aref;
+    uint32_t on = active_mask(jar);
+    LV term = ((0.5 * dd) * jar) * jar;
+    LV tb[GL];
+    bcast_all<X>(term, tb);
+#pragma unroll
+    for (int r = 0; r < GL; r++)
+      if (r < nr && ((on >> r) & 1u)) c += X::uni(tb[r]);
+    return c;
+  };
+
+  const LV a0 = X::gen([&](int l) { return l < nv ? LD(a)[l] : 0.0; });
+  LV a = a0;
+  {
+    const LV warm = X::gen([&](int l) { return l < nv ? LD(qacc)[l] : 0.0; });
+    Real c0 = cost(a0, a0), c1 = cost(warm, a0);
+    if (c1 < c0) a = warm;
+  }
+  const LV ma0 = mul_m(a0);
+  const Real scale = 1.0 + sqrt(X::uni(lsum_lanes<X>(ma0 * ma0, nv, -0.0)));
+  const Real tol = M.tolerance;
+  const int cap = M.iterations > 1 ? M.iterations : 1;
+  uint64_t prev = ~(uint64_t)0;
+  int iters = 0;
+  while (iters < cap) {
+    iters++;
+    const LV tmp = a - a0;
+    const LV jar = jdot(a) - aref;
+    const uint32_t act = active_mask(jar);
+    const LV grad = add_rows(mul_m(tmp), dd * jar, act);
+    const Real gnorm = sqrt(X::uni(lsum_lanes<X>(grad * grad, nv, -0.0)));
+    if (gnorm < tol * scale || ((uint64_t)act == prev && iters > 1)) break;
+    prev = act;
+    // H = M + sum_active D_r J_r^T J_r, row l in lane l
+    LV h[GL];
+#pragma unroll
+    for (int j = 0; j < GL; j++) {
+      h[j] = X::gen([&](int l) { return (l < nv && j < nv) ? qm[l * nv + j] : 0.0; });
+      h[j] = X::sel([&](int l) { return l == j; }, h[j] + 0.0, h[j]);
+    }
+    for (int r = 0; r < nr; r++) {
+      if (!((act >> r) & 1u)) continue;
+      const LV ji = jrow(r);
+      const LV di = E.dd[r] * ji;
+#pragma unroll
+      for (int j = 0; j < GL; j++)
+        if (j < nv) h[j] = X::sel([&](int l) { return X::at(ji, l) != 0.0; }, h[j] + di * J[r * nv + j], h[j]);
+    }
+    if (!chol_rows<X>(h, nv)) break;
+    store_lower<X>(h, Lf, nv, 0, nv);
+    X::sync();
+    LV lr[GL], lc[GL], lii;
+    load_factor<X>(Lf, nv, 0, nv, lr, lc, &lii);
+    const LV p = solve_rows<X>(lr, lc, lii, -grad, nv);
+    // exact line search along p (newton.rs:310-351)
+    const LV mp = mul_m(p);
+    const Real mpp = X::uni(lsum_lanes<X>(p * mp, nv, -0.0));
+    const Real mpg = X::uni(lsum_lanes<X>(tmp * mp, nv, -0.0));
+    const LV jp = jdot(p);
+    LV jarb[GL], jpb[GL];
+    bcast_all<X>(jar, jarb);
+    bcast_all<X>(jp, jpb);
+    auto deriv = [&](Real al, Real* d1o, Real* d2o) {
+      Real d1 = al * mpp + mpg, d2 = mpp;
+#pragma unroll
+      for (int r = 0; r < GL; r++)
+        if (r < nr) {
+          Real jpr = X::uni(jpb[r]);
+          Real v = X::uni(jarb[r]) + al * jpr;
+          if (r >= n_ineq || v < 0.0) {
+            Real dq = X::uni(ddb[r]) * jpr;
+            d1 += dq * v;
+            d2 += dq * jpr;
+          }
+        }
+      *d1o = d1;
+      *d2o = d2;
+    };
+    Real lo = 0.0, hi = INFINITY, al = 1.0, d1, d2;
+    deriv(0.0, &d1, &d2);
+    const Real ltol = 1e-12 * rmax(fabs(d1), 1e-30);
+    for (int it = 0; it < 50; it++) {
+      deriv(al, &d1, &d2);
+      if (fabs(d1) <= ltol) break;
+      if (d1 < 0.0) lo = al;
+      else hi = al;
+      Real next = al - d1 / rmax(d2, 1e-30);
+      if (next <= lo || next >= hi) next = rfinite(hi) ? 0.5 * (lo + hi) : 2.0 * rmax(al, lo);
+      if (fabs(next - al) <= 1e-14 * rmax(fabs(al), 1.0)) {
+        al = next;
+        break;
+      }
+      al = next;
+    }
+    a = a + al * p;
+  }
+  // add_constraint_force (newton.rs:361-371), from zero
+  const LV jar = jdot(a) - aref;
+  const LV f = add_rows(X::splat(0.0), (-dd) * jar, active_mask(jar));
+  X::each([&](int l) {
+    if (l < nv) {
+      LD(a)[l] = X::at(a, l);
+      LD(qfrc_c)[l] = X::at(f, l);
+    }
+  });
+  X::sync();
+  return iters;
+}
+
 template <class X>
 PHD int newton_dense(const DModel& M, Env& E, int nr) {
+  if (M.nv <= GL && nr <= GL && nr <= M.nrl) return newton_lanes<X>(M, E, nr);
   const int nv = M.nv;
   Real* a = LD(a);
   Real* a0 = LD(a0);

@@ -43,6 +43,9 @@ __device__ __forceinline__ DLV operator*(DLV a, DLV b) { return {a.v * b.v}; }
 __device__ __forceinline__ DLV operator/(DLV a, DLV b) { return {a.v / b.v}; }
 __device__ __forceinline__ DLV operator/(Real a, DLV b) { return {a / b.v}; }
 __device__ __forceinline__ DLV operator-(DLV a) { return {-a.v}; }
+__device__ __forceinline__ DLV operator*(Real a, DLV b) { return {a * b.v}; }
+__device__ __forceinline__ DLV operator*(DLV a, Real b) { return {a.v * b}; }
+__device__ __forceinline__ DLV operator+(DLV a, Real b) { return {a.v + b}; }
 
 struct DevLanes {
   using LV = DLV;
@@ -65,6 +68,12 @@ struct DevLanes {
   __device__ __forceinline__ static DLV sqrt_(DLV a) { return {sqrt(a.v)}; }
   // the value of a group-uniform vector
   __device__ __forceinline__ static Real uni(DLV a) { return a.v; }
+  // bit l set when pred holds on lane l of this lane's 16-lane row
+  template <class P>
+  __device__ __forceinline__ static uint32_t ballot(P&& pred) {
+    uint64_t m = __ballot(pred(lane()) ? 1 : 0);
+    return (uint32_t)((m >> (threadIdx.x & 48u)) & 0xFFFFu);
+  }
   template <int J>
   __device__ __forceinline__ static DLV bc(DLV x) {
     int lo = __double2loint(x.v), hi = __double2hiint(x.v);
@@ -99,6 +108,21 @@ inline HLV operator-(const HLV& a) {
   for (int l = 0; l < GL; l++) r.v[l] = -a.v[l];
   return r;
 }
+inline HLV operator*(Real a, const HLV& b) {
+  HLV r;
+  for (int l = 0; l < GL; l++) r.v[l] = a * b.v[l];
+  return r;
+}
+inline HLV operator*(const HLV& a, Real b) {
+  HLV r;
+  for (int l = 0; l < GL; l++) r.v[l] = a.v[l] * b;
+  return r;
+}
+inline HLV operator+(const HLV& a, Real b) {
+  HLV r;
+  for (int l = 0; l < GL; l++) r.v[l] = a.v[l] + b;
+  return r;
+}
 
 struct HostLanes {
   using LV = HLV;
@@ -130,6 +154,13 @@ struct HostLanes {
     return r;
   }
   static Real uni(const HLV& a) { return a.v[0]; }
+  template <class P>
+  static uint32_t ballot(P&& pred) {
+    uint32_t m = 0;
+    for (int l = 0; l < GL; l++)
+      if (pred(l)) m |= 1u << l;
+    return m;
+  }
   template <int J>
   static HLV bc(const HLV& x) {
     return splat(x.v[J]);
@@ -163,19 +194,65 @@ PHD LV lsdot(const LV* a, const LV* b) {
   }
 }
 
+// out[k] = lane k's x, in every lane
+template <class X, class LV>
+PHD void bcast_all(LV x, LV* out) {
+  sfor<0, GL>([&](auto KC) {
+    constexpr int k = decltype(KC)::value;
+    out[k] = X::template bc<k>(x);
+    return true;
+  });
+}
+
+// sdot_n (simd.rs:14-64) of a runtime length n <= 16 over register arrays;
+// the structure switches on n (uniform), the unrolled terms stay in order.
+template <class X, class LV>
+PHD LV lsdot_rt(const LV* a, const LV* b, int n) {
+  if (n < 8) {
+    LV s = X::splat(0.0);
+#pragma unroll
+    for (int k = 0; k < 7; k++)
+      if (k < n) s = s + a[k] * b[k];
+    return s;
+  }
+  LV s0 = X::splat(0.0), s1 = s0, s2 = s0, s3 = s0;
+  const int nb = n / 4 * 4;
+#pragma unroll
+  for (int k = 0; k < GL; k += 4)
+    if (k + 4 <= nb) {
+      s0 = s0 + a[k] * b[k];
+      s1 = s1 + a[k + 1] * b[k + 1];
+      s2 = s2 + a[k + 2] * b[k + 2];
+      s3 = s3 + a[k + 3] * b[k + 3];
+    }
+  LV t = (s0 + s1) + (s2 + s3);
+#pragma unroll
+  for (int k = 8; k < GL; k++)
+    if (k >= nb && k < n) t = t + a[k] * b[k];
+  return t;
+}
+
+// Sequential sum of lane values 0..n-1 starting from `start` (Rust's
+// iterator sum folds from -0.0), computed redundantly in every lane.
+template <class X, class LV>
+PHD LV lsum_lanes(LV x, int n, Real start) {
+  LV xb[GL];
+  bcast_all<X>(x, xb);
+  LV s = X::splat(start);
+#pragma unroll
+  for (int k = 0; k < GL; k++)
+    if (k < n) s = s + xb[k];
+  return s;
+}
+
 // math.rs:425-447 with one row per lane: in-place lower Cholesky of the
 // diagonal block [s0, s0 + n) (n <= 16) of the row-major stride-nv matrix A.
 // Column j: lanes receive row j (entries < j) by broadcast, each lane forms its
 // dot with row j, lane j's diagonal is broadcast, square-rooted and inverted,
 // and rows below scale their entry j. Only the lower triangle is written, as in
 // the reference. Returns false when a pivot is not positive.
-template <class X>
-PHD bool chol_lanes(Real* A, int nv, int s0, int n) {
-  using LV = typename X::LV;
-  LV r[GL];
-#pragma unroll
-  for (int k = 0; k < GL; k++)
-    r[k] = X::gen([&](int l) { return (l < n && k < n) ? A[(s0 + l) * nv + s0 + k] : 0.0; });
+template <class X, class LV>
+PHD bool chol_rows(LV* r, int n) {
   bool ok = true;
   sfor<0, GL>([&](auto JC) {
     constexpr int j = decltype(JC)::value;
@@ -194,7 +271,13 @@ PHD bool chol_lanes(Real* A, int nv, int s0, int n) {
     r[j] = X::sel([](int l) { return l == j; }, ljj, X::sel([](int l) { return l > j; }, d * inv, r[j]));
     return true;
   });
-  if (!ok) return false;
+  return ok;
+}
+
+// Store the lower triangle (with the diagonal) of register rows into the
+// diagonal block [s0, s0 + n) of A.
+template <class X, class LV>
+PHD void store_lower(const LV* r, Real* A, int nv, int s0, int n) {
   X::each([&](int l) {
     if (l < n) {
 #pragma unroll
@@ -202,6 +285,17 @@ PHD bool chol_lanes(Real* A, int nv, int s0, int n) {
         if (k <= l) A[(s0 + l) * nv + s0 + k] = X::at(r[k], l);
     }
   });
+}
+
+template <class X>
+PHD bool chol_lanes(Real* A, int nv, int s0, int n) {
+  using LV = typename X::LV;
+  LV r[GL];
+#pragma unroll
+  for (int k = 0; k < GL; k++)
+    r[k] = X::gen([&](int l) { return (l < n && k < n) ? A[(s0 + l) * nv + s0 + k] : 0.0; });
+  if (!chol_rows<X>(r, n)) return false;
+  store_lower<X>(r, A, nv, s0, n);
   return true;
 }
 
@@ -209,17 +303,9 @@ PHD bool chol_lanes(Real* A, int nv, int s0, int n) {
 // its dot with the solved prefix, divides, and broadcasts), then backward
 // substitution (lane i divides and broadcasts; lanes above update with
 // column entries they loaded up front). x is overwritten with the solution.
-template <class X>
-PHD void solve_lanes(const Real* L, int nv, int s0, int n, Real* x) {
-  using LV = typename X::LV;
-  LV r[GL], c[GL], xs[GL];
-#pragma unroll
-  for (int k = 0; k < GL; k++) {
-    r[k] = X::gen([&](int l) { return (l < n && k < l) ? L[(s0 + l) * nv + s0 + k] : 0.0; });
-    c[k] = X::gen([&](int l) { return (k < n && k > l) ? L[(s0 + k) * nv + s0 + l] : 0.0; });
-  }
-  LV lii = X::gen([&](int l) { return l < n ? L[(s0 + l) * nv + s0 + l] : 1.0; });
-  LV xv = X::gen([&](int l) { return l < n ? x[s0 + l] : 0.0; });
+template <class X, class LV>
+PHD LV solve_rows(const LV* r, const LV* c, LV lii, LV xv, int n) {
+  LV xs[GL];
   sfor<0, GL>([&](auto IC) {
     constexpr int i = decltype(IC)::value;
     if (i >= n) return false;
@@ -235,6 +321,28 @@ PHD void solve_lanes(const Real* L, int nv, int s0, int n, Real* x) {
     LV xi = X::template bc<i>(q);
     xv = X::sel([](int l) { return l == i; }, q, X::sel([](int l) { return l < i; }, xv + (-xi) * c[i], xv));
   });
+  return xv;
+}
+
+// Load, for lane l, row l of the strict lower triangle (r), column l below the
+// diagonal (c) and the diagonal entry of the factor in block [s0, s0 + n).
+template <class X, class LV>
+PHD void load_factor(const Real* L, int nv, int s0, int n, LV* r, LV* c, LV* lii) {
+#pragma unroll
+  for (int k = 0; k < GL; k++) {
+    r[k] = X::gen([&](int l) { return (l < n && k < l) ? L[(s0 + l) * nv + s0 + k] : 0.0; });
+    c[k] = X::gen([&](int l) { return (k < n && k > l) ? L[(s0 + k) * nv + s0 + l] : 0.0; });
+  }
+  *lii = X::gen([&](int l) { return l < n ? L[(s0 + l) * nv + s0 + l] : 1.0; });
+}
+
+template <class X>
+PHD void solve_lanes(const Real* L, int nv, int s0, int n, Real* x) {
+  using LV = typename X::LV;
+  LV r[GL], c[GL], lii;
+  load_factor<X>(L, nv, s0, n, r, c, &lii);
+  LV xv = X::gen([&](int l) { return l < n ? x[s0 + l] : 0.0; });
+  xv = solve_rows<X>(r, c, lii, xv, n);
   X::each([&](int l) {
     if (l < n) x[s0 + l] = X::at(xv, l);
   });
