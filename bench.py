@@ -89,6 +89,21 @@ def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
     }, ops
 
 
+def copy_calibration(gib: float) -> None:
+    """Five device-to-device copies of `gib` GiB: the known-size copy that HBM
+    byte counters (FETCH_SIZE, WRITE_SIZE) are scaled against."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,9 +112,18 @@ def main():
     ap.add_argument("--envs", type=int, default=16384)
     ap.add_argument("--substeps", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-envs", type=int, default=4096, help="envs in the CPU-baseline sample")
+    ap.add_argument("--cpu-envs", type=int, default=16384, help="envs in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--iter-stats", type=int, default=20,
+                    help="extra (untimed) actions stepped one physics step at a time for the Newton "
+                         "iteration and contact statistics; 0 skips them (profiling runs)")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device-to-device copy of this many GiB five times "
+                         "(reference kernel for scaling HBM byte counters), then exit")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X and the environments are "
               f"independent (zero exchange while stepping), so --gpus {args.gpus} is not run.", file=sys.stderr)
@@ -139,14 +163,14 @@ def main():
 
     # Newton iterations per step and contacts, over a few more (untimed) actions
     iters, ncon = [], []
-    for a in range(nact - min(20, nact), nact):
+    for a in range(nact - min(args.iter_stats, nact), nact):
         sim.set_device("ctrl", 0, n, base + a * stride)
         for _ in range(sub):
             sim.step(0, n, 1)
             iters.append(sim.solver_iterations())
             ncon.append(sim.ncon())
-    iters = np.concatenate(iters)
-    ncon = np.concatenate(ncon)
+    iters = np.concatenate(iters) if iters else np.zeros(1)
+    ncon = np.concatenate(ncon) if ncon else np.zeros(1, dtype=np.int32)
 
     cb, ops = (None, OPS_PER_ENV_STEP_DEFAULT) if args.no_cpu else cpu_baseline(
         arrays, q0, ctrl, sub, args.warmup, args.cpu_envs)

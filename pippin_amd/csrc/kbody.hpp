@@ -1166,7 +1166,7 @@ PHD void store_poses(const DModel& M, const DState& S, Env& E) {
   X::par(9 * M.nbody, [&](int k) { S.xmat[e * 9 * M.nbody + k] = s[M.L.xmat + k]; });
 }
 
-// Load one env, run nstep steps, refresh its poses and store it back.
+// Load one env, run nstep steps and store it back.
 template <class X>
 PHD void run_env(const DModel& M, const DState& S, Env& E, int64_t nstep, const Real* ctrl_seq, Real* qpos_out,
                  int64_t rel) {
@@ -1180,8 +1180,7 @@ PHD void run_env(const DModel& M, const DState& S, Env& E, int64_t nstep, const 
     if (!step_env<X>(M, S, E)) break;
     if (qpos_out) X::par(M.nq, [&](int k) { qpos_out[(rel * nstep + t) * M.nq + k] = lds[M.L.qpos + k]; });
   }
-  kinematics<X>(M, E);
-  store_poses<X>(M, S, E);
+  // poses are not refreshed here: they are derived from qpos on demand (k_kin)
   X::par(M.nq, [&](int k) { S.qpos[e * M.nq + k] = lds[M.L.qpos + k]; });
   X::par(M.nv, [&](int k) {
     S.qvel[e * M.nv + k] = lds[M.L.qvel + k];

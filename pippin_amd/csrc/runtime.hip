@@ -191,6 +191,10 @@ struct ph_sim {
   int64_t n = 0;
   int device = 0;
   int envs_per_wg = 1;  // environments packed into one k_step workgroup
+  // Poses are a function of qpos (PippinCpu reruns kinematics after step and
+  // set(Qpos), pippin_cpu.rs:107-110, 183-185); they are recomputed for all
+  // envs on the first poses() call after qpos changed.
+  mutable bool poses_stale = true;
   hipStream_t stream = nullptr;
   DModel M;
   DState S;
@@ -495,10 +499,6 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   if (int rc = upload_variants(s.get())) return rc;
   if (num_envs > 0) {
     if (int rc = sync_desc(s.get())) return rc;
-    hipLaunchKernelGGL(k_kin, dim3((unsigned)num_envs), dim3(64), M.L.bytes, s->stream, s->d_model.p, s->d_state.p,
-                       (int64_t)0, (const int64_t*)nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s->stream));
   }
   *out = s.release();
   return PH_OK;
@@ -582,6 +582,7 @@ extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t ns
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipEventRecord(s->ev0, s->stream));
   launch_step(s, begin, end, nstep, nullptr, nullptr);
+  s->poses_stale = true;
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -620,6 +621,7 @@ extern "C" int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, 
   if (end == begin) return PH_OK;
   HIPCHK(hipSetDevice(s->device));
   launch_step(s, begin, end, nstep, ctrl, qpos_out);
+  s->poses_stale = true;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   return status_of(s, begin, end);
@@ -653,12 +655,7 @@ extern "C" int pippin_hip_set(ph_sim* s, int32_t field, int64_t begin, int64_t e
   if (int rc = check_len("set", len, end - begin, w)) return rc;
   if (w == 0 || end == begin) return PH_OK;
   HIPCHK(hipMemcpy(d + begin * w, src, (end - begin) * w * sizeof(double), hipMemcpyHostToDevice));
-  if (field == PH_FIELD_QPOS) {
-    hipLaunchKernelGGL(k_kin, dim3((unsigned)(end - begin)), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, begin,
-                       (const int64_t*)nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s->stream));
-  }
+  if (field == PH_FIELD_QPOS) s->poses_stale = true;
   return PH_OK;
 }
 
@@ -671,9 +668,8 @@ extern "C" int pippin_hip_reset(ph_sim* s, const int64_t* envs, int64_t n) {
   HIPCHK(s->d_ids.upload(ids));
   hipLaunchKernelGGL(k_reset, dim3((unsigned)n), dim3(64), 0, s->stream, s->d_model.p, s->d_state.p, s->d_ids.p);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_kin, dim3((unsigned)n), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p, (int64_t)0, s->d_ids.p);
-  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
+  s->poses_stale = true;
   return PH_OK;
 }
 
@@ -685,6 +681,14 @@ extern "C" int pippin_hip_poses(const ph_sim* s, int64_t begin, int64_t end, ph_
   if (int rc = check_len("poses(cams)", ncl, end - begin, d.ncam)) return rc;
   int64_t n = end - begin;
   if (n == 0) return PH_OK;
+  if (s->poses_stale) {
+    HIPCHK(hipSetDevice(s->device));
+    hipLaunchKernelGGL(k_kin, dim3((unsigned)s->n), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p,
+                       (int64_t)0, (const int64_t*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->poses_stale = false;
+  }
   std::vector<double> gp(n * 3 * d.ngeom), gm(n * 9 * d.ngeom), bp(n * 3 * d.nbody), bm(n * 9 * d.nbody);
   if (d.ngeom) {
     HIPCHK(hipMemcpy(gp.data(), s->s_gxpos.p + begin * 3 * d.ngeom, gp.size() * 8, hipMemcpyDeviceToHost));

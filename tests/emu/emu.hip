@@ -303,6 +303,16 @@ int emu_invweight(emu_sim* s, double* dof, double* body) {
 
 int emu_poses(emu_sim* s, int64_t begin, int64_t end, double* gxpos, double* gxmat) {
   int ng = s->M.ngeom;
+  for (int64_t e = begin; e < end; e++) {  // poses on demand, as the device library computes them
+    Env E;
+    E.s = s->lds.data();
+    E.si = reinterpret_cast<int*>(E.s + s->M.L.ndouble);
+    E.e = e;
+    E.var = s->env_var[e];
+    load_state<HostX>(s->M, s->S, E);
+    kinematics<HostX>(s->M, E);
+    store_poses<HostX>(s->M, s->S, E);
+  }
   memcpy(gxpos, s->gxpos.data() + begin * 3 * ng, (end - begin) * 3 * ng * sizeof(double));
   memcpy(gxmat, s->gxmat.data() + begin * 9 * ng, (end - begin) * 9 * ng * sizeof(double));
   return 0;
