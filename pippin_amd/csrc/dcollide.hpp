@@ -15,10 +15,10 @@ struct GPose {
 };
 
 struct Sink {
-  Real* out;  // MAXHIT x CW
+  gp<Real> out;  // MAXHIT x CW, in global memory
   int n;
   PHD void push(V3 pos, V3 nrm, Real depth) {
-    Real* h = out + n * CW;
+    gp<Real> h = out + n * CW;
     st3(h, pos);
     st3(h + 3, nrm);
     st3(h + 6, v3z());
@@ -478,7 +478,7 @@ PHD void collide_pair(const GPose& a, const GPose& b, Real margin, Sink& out) {
     int start = out.n;
     collide_ordered(b, a, margin, out);
     for (int i = start; i < out.n; i++) {
-      Real* h = out.out + i * CW + 3;
+      gp<Real> h = out.out + (i * CW + 3);
       h[0] = -h[0];
       h[1] = -h[1];
       h[2] = -h[2];

@@ -175,8 +175,10 @@ PHD V3 normalized(V3 a) {
 PHD V3 vabs(V3 a) { return v3(fabs(a.x), fabs(a.y), fabs(a.z)); }
 PHD V3 mul_elem(V3 a, V3 o) { return v3(a.x * o.x, a.y * o.y, a.z * o.z); }
 PHD V3 vmax(V3 a, V3 o) { return v3(rmax(a.x, o.x), rmax(a.y, o.y), rmax(a.z, o.z)); }
-PHD V3 ld3(const Real* p) { return v3(p[0], p[1], p[2]); }
-PHD void st3(Real* p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+template <class P>
+PHD V3 ld3(P p) { return v3(p[0], p[1], p[2]); }
+template <class P>
+PHD void st3(P p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
 PHD void tangents(V3 n, V3* t1, V3* t2) {
   V3 a = fabs(n.x) < 0.57 ? v3(1, 0, 0) : v3(0, 1, 0);
   *t1 = normalized(cross(n, a));
@@ -188,12 +190,14 @@ struct M3 {
   Real m[9];
 };
 PHD M3 m3id() { M3 r = {{1, 0, 0, 0, 1, 0, 0, 0, 1}}; return r; }
-PHD M3 ldm(const Real* p) {
+template <class P>
+PHD M3 ldm(P p) {
   M3 r;
   for (int i = 0; i < 9; i++) r.m[i] = p[i];
   return r;
 }
-PHD void stm(Real* p, const M3& r) {
+template <class P>
+PHD void stm(P p, const M3& r) {
   for (int i = 0; i < 9; i++) p[i] = r.m[i];
 }
 PHD V3 col(const M3& a, int j) { return v3(a.m[j], a.m[3 + j], a.m[6 + j]); }
@@ -241,8 +245,10 @@ struct Q4 {
   Real w, x, y, z;
 };
 PHD Q4 q4(Real w, Real x, Real y, Real z) { Q4 r; r.w = w; r.x = x; r.y = y; r.z = z; return r; }
-PHD Q4 ldq(const Real* p) { return q4(p[0], p[1], p[2], p[3]); }
-PHD void stq(Real* p, Q4 q) { p[0] = q.w; p[1] = q.x; p[2] = q.y; p[3] = q.z; }
+template <class P>
+PHD Q4 ldq(P p) { return q4(p[0], p[1], p[2], p[3]); }
+template <class P>
+PHD void stq(P p, Q4 q) { p[0] = q.w; p[1] = q.x; p[2] = q.y; p[3] = q.z; }
 PHD Q4 qnormalized(Q4 q) {
   Real s = rsum0();
   s += q.w * q.w;
@@ -284,8 +290,10 @@ struct SP {
   V3 a, l;
 };
 PHD SP sp(V3 a, V3 l) { SP s; s.a = a; s.l = l; return s; }
-PHD SP ldsp(const Real* p) { return sp(ld3(p), ld3(p + 3)); }
-PHD void stsp(Real* p, const SP& s) { st3(p, s.a); st3(p + 3, s.l); }
+template <class P>
+PHD SP ldsp(P p) { return sp(ld3(p), ld3(p + 3)); }
+template <class P>
+PHD void stsp(P p, const SP& s) { st3(p, s.a); st3(p + 3, s.l); }
 PHD Real sdot(const SP& a, const SP& o) { return dot(a.a, o.a) + dot(a.l, o.l); }
 PHD SP sscale(const SP& a, Real s) { return sp(a.a * s, a.l * s); }
 PHD SP spadd(const SP& a, const SP& o) { return sp(a.a + o.a, a.l + o.l); }
@@ -294,7 +302,8 @@ PHD SP cross_force(const SP& s, const SP& f) { return sp(cross(s.a, f.a) + cross
 PHD V3 point_velocity(const SP& s, V3 p) { return s.l + cross(s.a, p); }
 
 // Spatial inertia {mass, m*c, I_O} stored as 13 doubles: [mass, mc(3), I(9)].
-PHD SP mul_motion(const Real* I, const SP& v) {
+template <class P>
+PHD SP mul_motion(P I, const SP& v) {
   M3 io = ldm(I + 4);
   V3 mc = ld3(I + 1);
   Real mass = I[0];
@@ -302,7 +311,8 @@ PHD SP mul_motion(const Real* I, const SP& v) {
 }
 
 // simd.rs:14-64 (x86 branch), over a strided pointer.
-PHD Real sdot_n(const Real* a, const Real* b, int n) {
+template <class PA, class PB>
+PHD Real sdot_n(PA a, PB b, int n) {
   if (n < 8) {
     Real s = 0.0;
     for (int k = 0; k < n; k++) s += a[k] * b[k];

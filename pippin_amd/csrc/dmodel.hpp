@@ -8,6 +8,49 @@
 
 namespace phk {
 
+// Pointers into device buffers. In device code every access goes through an
+// explicit address space, never a generic (flat) pointer: flat accesses also
+// count against the LDS wait counter, which would make every LDS wait also
+// wait for HBM. Read-only model data (never written while a kernel runs) is
+// read through the constant address space, so loads at wave-uniform indices
+// become scalar loads; state buffers are read and written as global memory.
+// On the host both are plain pointers; the layout is one pointer either way.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PH_AS_GLOBAL __attribute__((address_space(1)))
+#define PH_AS_CONST __attribute__((address_space(4)))
+#else
+#define PH_AS_GLOBAL
+#define PH_AS_CONST
+#endif
+template <class T>
+struct gp {
+  T* p;
+  gp& operator=(T* q) {
+    p = q;
+    return *this;
+  }
+  PHD PH_AS_GLOBAL T& operator[](int64_t i) const { return ((PH_AS_GLOBAL T*)p)[i]; }
+  PHD gp operator+(int64_t i) const {
+    gp r;
+    r.p = p + i;
+    return r;
+  }
+};
+template <class T>
+struct gp<const T> {
+  const T* p;
+  gp& operator=(const T* q) {
+    p = q;
+    return *this;
+  }
+  PHD T operator[](int64_t i) const { return ((const PH_AS_CONST T*)p)[i]; }
+  PHD gp operator+(int64_t i) const {
+    gp r;
+    r.p = p + i;
+    return r;
+  }
+};
+
 enum { J_FREE = 0, J_BALL = 1, J_SLIDE = 2, J_HINGE = 3 };
 enum { G_PLANE = 0, G_SPHERE = 1, G_CAPSULE = 2, G_BOX = 3, G_CYLINDER = 4, G_MESH = 5 };
 
@@ -39,44 +82,45 @@ struct Layout {
 };
 
 struct DModel {
-  int nq, nv, nu, nbody, njnt, ngeom, ncam, neq, npair, nlevel, nblock;
+  int nq, nv, nu, nbody, njnt, ngeom, ncam, neq, npair, nlevel, nblock, nacc;
   int integrator, solver_kind, iterations;
   Real timestep, gravity[3], tolerance, margin, solref[2], solimp[5], impratio, hertz, zeta, slop;
   int ncmax;  // contact capacity per env (the sum of per-pair maxima: never exceeded)
   int nrmax;  // row capacity per env (4 per contact for Newton, 3 for PGS, + limits + equalities)
   int nrl;    // rows kept in LDS; larger problems use the global row buffers
 
-  const int *body_parent, *body_jntadr, *body_jntnum, *body_dofadr, *body_dofnum, *body_lastdof;
-  const uint64_t* body_chain;  // dofs on the path world..body (bit k = dof k)
-  const Real *body_pos, *body_quat;
-  const int *level_adr, *level_body;  // bodies grouped by depth (parents first)
-  const int *jnt_type, *jnt_qposadr, *jnt_dofadr, *jnt_limited;
-  const Real *jnt_pos, *jnt_axis, *jnt_range, *jnt_stiffness, *jnt_springref;
-  const int *dof_body, *dof_parent, *dof_jnt;
-  const uint64_t* dof_anc;  // strict ancestors of each dof
-  const Real* dof_armature;
-  const int* block_adr;  // nblock + 1 dof boundaries
-  const int *geom_type, *geom_body;
-  const Real *geom_pos, *geom_quat;
-  const int *pair_g1, *pair_g2;
-  const Real* act_gear;
-  const int *act_ctrllimited, *act_forcelimited;
-  const Real *act_ctrlrange, *act_forcerange;
-  const int *act_madr, *act_mnum, *mom_jnt;
-  const Real* mom_coef;
-  const int *dofact_adr, *dofact_num, *dofact_u;  // per dof: actuator contributions in (u, moment) order
-  const Real* dofact_coef;
-  const int *eq_j1, *eq_j2;
-  const Real *eq_poly, *eq_solref, *eq_solimp;
-  const Real* qpos0;
+  gp<const int> body_parent, body_jntadr, body_jntnum, body_dofadr, body_dofnum, body_lastdof;
+  gp<const uint64_t> body_chain;  // dofs on the path world..body (bit k = dof k)
+  gp<const Real> body_pos, body_quat;
+  gp<const int> level_adr, level_body;  // bodies grouped by depth (parents first)
+  gp<const int> acc_level_adr, acc_parent, acc_child_adr, acc_child;  // leaf-to-root sum schedule
+  gp<const int> jnt_type, jnt_qposadr, jnt_dofadr, jnt_limited;
+  gp<const Real> jnt_pos, jnt_axis, jnt_range, jnt_stiffness, jnt_springref;
+  gp<const int> dof_body, dof_parent, dof_jnt;
+  gp<const uint64_t> dof_anc;  // strict ancestors of each dof
+  gp<const Real> dof_armature;
+  gp<const int> block_adr;  // nblock + 1 dof boundaries
+  gp<const int> geom_type, geom_body;
+  gp<const Real> geom_pos, geom_quat;
+  gp<const int> pair_g1, pair_g2;
+  gp<const Real> act_gear;
+  gp<const int> act_ctrllimited, act_forcelimited;
+  gp<const Real> act_ctrlrange, act_forcerange;
+  gp<const int> act_madr, act_mnum, mom_jnt;
+  gp<const Real> mom_coef;
+  gp<const int> dofact_adr, dofact_num, dofact_u;  // per dof: actuator contributions in (u, moment) order
+  gp<const Real> dofact_coef;
+  gp<const int> eq_j1, eq_j2;
+  gp<const Real> eq_poly, eq_solref, eq_solimp;
+  gp<const Real> qpos0;
 
   // per-variant parameters (params.rs), variant of env e = env_var[e]
-  const int* env_var;
-  const Real *v_geom_size, *v_geom_aabb, *v_friction;
-  const uint32_t *v_contype, *v_conaffinity;
-  const Real *v_body_mass, *v_body_ipos, *v_body_inertia, *v_dof_damping, *v_act_gain, *v_act_bias;
-  const Real *v_dof_invweight, *v_body_invweight, *v_qpos_reset;
-  const int* v_damped;
+  gp<const int> env_var;
+  gp<const Real> v_geom_size, v_geom_aabb, v_friction;
+  gp<const uint32_t> v_contype, v_conaffinity;
+  gp<const Real> v_body_mass, v_body_ipos, v_body_inertia, v_dof_damping, v_act_gain, v_act_bias;
+  gp<const Real> v_dof_invweight, v_body_invweight, v_qpos_reset;
+  gp<const int> v_damped;
 
   Layout L;
 };
@@ -84,18 +128,18 @@ struct DModel {
 // Per-env state and scratch in global memory; env e owns slice e of each.
 struct DState {
   int64_t n;
-  Real *qpos, *qvel, *ctrl, *qacc, *applied, *time;
-  Real *geom_xpos, *geom_xmat, *xpos, *xmat;  // poses after the last step
-  int *ncon, *iters, *nrows, *status;
-  Real* con;       // n x ncmax x CW
-  int* cgeom;      // n x ncmax x 3 (g1, g2, feature)
-  Real* pcon;      // PGS history: previous contacts
-  int* pcgeom;
-  Real* plambda;   // n x ncmax x 3
-  int* npcon;
-  Real* stage;     // n x npair x MAXHIT x CW
-  Real* gjac;      // n x nrmax x nv
-  Real* grow;      // n x nrmax x 10
+  gp<Real> qpos, qvel, ctrl, qacc, applied, time;
+  gp<Real> geom_xpos, geom_xmat, xpos, xmat;  // poses after the last step
+  gp<int> ncon, iters, nrows, status;
+  gp<Real> con;       // n x ncmax x CW
+  gp<int> cgeom;      // n x ncmax x 3 (g1, g2, feature)
+  gp<Real> pcon;      // PGS history: previous contacts
+  gp<int> pcgeom;
+  gp<Real> plambda;   // n x ncmax x 3
+  gp<int> npcon;
+  gp<Real> stage;     // n x npair x MAXHIT x CW
+  gp<Real> gjac;      // n x nrmax x nv
+  gp<Real> grow;      // n x nrmax x 10
 };
 
 }  // namespace phk

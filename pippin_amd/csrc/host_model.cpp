@@ -283,6 +283,29 @@ bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
       if (depth[b] == l) m.level_body.push_back(b);
   }
   m.level_adr.push_back((int)m.level_body.size());
+  // Leaf-to-root sums (forward.rs:100-103, 166-170: for b descending,
+  // x[parent(b)] += x[b]). A parent's sum only depends on the order of its own
+  // children, which all sit one level below it; so levels deepest first, and
+  // within a level each parent adds its children in descending index order.
+  m.acc_level_adr.clear();
+  m.acc_parent.clear();
+  m.acc_child_adr.clear();
+  m.acc_child.clear();
+  for (int l = maxd; l >= 1; l--) {
+    m.acc_level_adr.push_back((int)m.acc_parent.size());
+    std::vector<int> parents;
+    for (int b = 1; b < nb; b++)
+      if (depth[b] == l && std::find(parents.begin(), parents.end(), m.body_parent[b]) == parents.end())
+        parents.push_back(m.body_parent[b]);
+    for (int p : parents) {
+      m.acc_parent.push_back(p);
+      m.acc_child_adr.push_back((int)m.acc_child.size());
+      for (int b = nb - 1; b >= 1; b--)
+        if (depth[b] == l && m.body_parent[b] == p) m.acc_child.push_back(b);
+    }
+  }
+  m.acc_level_adr.push_back((int)m.acc_parent.size());
+  m.acc_child_adr.push_back((int)m.acc_child.size());
   // local boxes
   m.geom_aabb.assign(6 * ng, 0.0);
   for (int g = 0; g < ng; g++) local_aabb(m.geom_type[g], &m.geom_size[3 * g], &m.geom_aabb[6 * g]);
@@ -340,6 +363,7 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
   M.neq = d.neq;
   M.npair = (int)hm.pair_g1.size();
   M.nlevel = (int)hm.level_adr.size() - 1;
+  M.nacc = (int)hm.acc_level_adr.size() - 1;
   M.nblock = hm.block_adr.empty() ? 0 : (int)hm.block_adr.size() - 1;
   M.integrator = d.integrator;
   M.solver_kind = solver;

@@ -31,6 +31,9 @@ struct HostModel {
   std::vector<int32_t> body_lastdof, dof_parent, body_weldid, block_adr;
   std::vector<uint64_t> body_chain, dof_anc;
   std::vector<int32_t> level_adr, level_body;
+  // leaf-to-root accumulation schedule (crba / rne): levels deepest first;
+  // each level lists (parent, children in descending index order) groups
+  std::vector<int32_t> acc_level_adr, acc_parent, acc_child_adr, acc_child;
   std::vector<double> geom_aabb;  // ngeom x 6 (center, half-extents)
   std::vector<int32_t> pair_g1, pair_g2;  // candidate pairs: every static filter except the masks
   std::vector<int32_t> dofact_adr, dofact_num, dofact_u;

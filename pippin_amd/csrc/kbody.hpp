@@ -42,8 +42,8 @@ PHD void rows_bind(const DModel& M, const DState& S, Env& E, int nr) {
     E.rprev = LD(rprev);
     E.rf = LD(rf);
   } else {
-    E.J = S.gjac + E.e * (int64_t)M.nrmax * M.nv;
-    Real* g = S.grow + E.e * (int64_t)M.nrmax * 10;
+    E.J = (S.gjac + E.e * (int64_t)M.nrmax * M.nv).p;
+    Real* g = (S.grow + E.e * (int64_t)M.nrmax * 10).p;
     E.aref = g;
     E.dd = g + M.nrmax;
     E.jar = g + 2 * M.nrmax;
@@ -113,6 +113,15 @@ PHD void fk_body(const DModel& M, Env& E, int b) {
   st3(xpos + 3 * b, pos);
   stq(xquat + 4 * b, quat);
   stm(xmat + 9 * b, xm);
+}
+
+// Body centre of mass and composite-ready spatial inertia (the tail of each
+// body's step in forward.rs:14-90). They depend only on the body's own pose,
+// not on the tree order, so all bodies are done at once after the levels.
+template <class X>
+PHD void body_inertia(const DModel& M, Env& E, int b) {
+  V3 pos = ld3(LD(xpos) + 3 * b);
+  M3 xm = ldm(LD(xmat) + 9 * b);
   V3 xipos = pos + mul_vec(xm, ld3(M.v_body_ipos + ((int64_t)E.var * M.nbody + b) * 3));
   st3(LD(xipos) + 3 * b, xipos);
   // SpatialInertia::from_com(mass, xipos, I.rotate(xmat)), math.rs:388-393
@@ -148,6 +157,7 @@ PHD void kinematics(const DModel& M, Env& E) {
     int b0 = M.level_adr[l], nb = M.level_adr[l + 1] - b0;
     X::par(nb, [&](int i) { fk_body<X>(M, E, M.level_body[b0 + i]); });
   }
+  X::par(M.nbody - 1, [&](int i) { body_inertia<X>(M, E, i + 1); });
 }
 
 // Geom world poses, the last loop of forward.rs:14-90. Evaluated where they
@@ -163,6 +173,22 @@ PHD void geom_poses(const DModel& M, Env& E) {
   });
 }
 
+// x[parent(b)] += x[b] for b from the last body down to 1 (w components per
+// body), level by level: lanes take (parent, component) pairs of one level.
+template <class X>
+PHD void leaf_to_root(const DModel& M, Real* x, int w) {
+  for (int L = 0; L < M.nacc; L++) {
+    int g0 = M.acc_level_adr[L], ng = M.acc_level_adr[L + 1] - g0;
+    X::par(ng * w, [&](int t) {
+      int g = g0 + t / w, c = t % w;
+      int p = M.acc_parent[g];
+      Real v = x[w * p + c];
+      for (int k = M.acc_child_adr[g]; k < M.acc_child_adr[g + 1]; k++) v = v + x[w * M.acc_child[k] + c];
+      x[w * p + c] = v;
+    });
+  }
+}
+
 // ---------------- CRBA, forward.rs:93-112 ----------------
 template <class X>
 PHD void crba(const DModel& M, Env& E) {
@@ -172,13 +198,8 @@ PHD void crba(const DModel& M, Env& E) {
   Real* qm = LD(qm);
   // composite inertia leaf to root: each of the 13 components is its own
   // sequence of additions in the reference's (decreasing body) order
-  X::par(13, [&](int c) {
-    for (int b = 0; b < nb; b++) crb[13 * b + c] = ci[13 * b + c];
-    for (int b = nb - 1; b >= 1; b--) {
-      int p = M.body_parent[b];
-      crb[13 * p + c] = crb[13 * p + c] + crb[13 * b + c];
-    }
-  });
+  X::par(13 * nb, [&](int t) { crb[t] = ci[t]; });
+  leaf_to_root<X>(M, crb, 13);
   X::par(nv * nv, [&](int k) { qm[k] = 0.0; });
   X::par(nv, [&](int i) {
     const Real* cd = LD(cdof);
@@ -243,12 +264,7 @@ PHD void velocity_rne(const DModel& M, Env& E) {
       stsp(cfrc + 6 * b, spadd(mul_motion(ci + 13 * b, a), cross_force(v, h)));
     });
   }
-  X::par(6, [&](int c) {
-    for (int b = M.nbody - 1; b >= 1; b--) {
-      int p = M.body_parent[b];
-      cfrc[6 * p + c] = cfrc[6 * p + c] + cfrc[6 * b + c];
-    }
-  });
+  leaf_to_root<X>(M, cfrc, 6);
   X::par(M.nv, [&](int k) {
     LD(bias)[k] = sdot(ldsp(LD(cdof) + 6 * k), ldsp(cfrc + 6 * M.dof_body[k]));
   });
@@ -276,7 +292,7 @@ PHD void passive_actuation(const DModel& M, Env& E) {
     Real length = gear * sl, vel = gear * sv;
     Real ctrl = LD(ctrl)[u];
     if (M.act_ctrllimited[u]) ctrl = rclamp(ctrl, M.act_ctrlrange[2 * u], M.act_ctrlrange[2 * u + 1]);
-    const Real* b = M.v_act_bias + ((int64_t)E.var * M.nu + u) * 3;
+    auto b = M.v_act_bias + ((int64_t)E.var * M.nu + u) * 3;
     Real f = M.v_act_gain[(int64_t)E.var * M.nu + u] * ctrl + b[0] + b[1] * length + b[2] * vel;
     Real clamped = 0.0;
     if (M.act_forcelimited[u]) {
@@ -306,7 +322,7 @@ PHD void collide(const DModel& M, const DState& S, Env& E) {
   Real* aabb = LD(aabb);
   const Real margin = M.margin;
   X::par(ng, [&](int g) {
-    const Real* la = M.v_geom_aabb + ((int64_t)E.var * ng + g) * 6;
+    auto la = M.v_geom_aabb + ((int64_t)E.var * ng + g) * 6;
     V3 c = ld3(la), h = ld3(la + 3);
     const Real* r = LD(gxmat) + 9 * g;
     if (!rfinite(h.x)) {
@@ -321,12 +337,12 @@ PHD void collide(const DModel& M, const DState& S, Env& E) {
            fabs(r[6]) * h.x + fabs(r[7]) * h.y + fabs(r[8]) * h.z));
   });
   int* cnt = reinterpret_cast<int*>(LD(paircnt));
-  Real* stage = S.stage + E.e * (int64_t)M.npair * MAXHIT * CW;
+  auto stage = S.stage + E.e * (int64_t)M.npair * MAXHIT * CW;
   X::par(M.npair, [&](int p) {
     int g1 = M.pair_g1[p], g2 = M.pair_g2[p];
     cnt[p] = 0;
-    const uint32_t* ct = M.v_contype + (int64_t)E.var * ng;
-    const uint32_t* ca = M.v_conaffinity + (int64_t)E.var * ng;
+    auto ct = M.v_contype + (int64_t)E.var * ng;
+    auto ca = M.v_conaffinity + (int64_t)E.var * ng;
     if ((ct[g1] & ca[g2]) == 0 && (ct[g2] & ca[g1]) == 0) return;  // per-env masks, model.rs:439-443
     const Real* A = aabb + 6 * g1;
     const Real* B = aabb + 6 * g2;
@@ -347,30 +363,52 @@ PHD void collide(const DModel& M, const DState& S, Env& E) {
     collide_pair(a, b, margin, sink);
     cnt[p] = sink.n;
   });
-  // pair-ordered compaction (contacts are listed in pair order, then hit order)
+  // pair-ordered compaction (contacts are listed in pair order, then hit
+  // order): each lane of the 16-lane row sums a contiguous run of pairs, the
+  // run totals are scanned across lanes, then each lane writes its offsets
   int* off = reinterpret_cast<int*>(LD(pairoff));
   int* isc = E.si + M.L.i_iscal;
-  X::par(1, [&](int) {
-    int o = 0;
-    for (int p = 0; p < M.npair; p++) {
-      off[p] = o;
-      o += cnt[p];
-    }
-    if (o > M.ncmax) {  // only reachable with a user-lowered contact capacity
-      isc[7] |= ST_CONTACT_OVERFLOW;
-      o = M.ncmax;
-    }
-    isc[0] = o;  // ncon
-  });
-  Real* con = S.con + E.e * (int64_t)M.ncmax * CW;
-  int* cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
+  {
+    using LV = typename X::LV;
+    const int np = M.npair, run = (np + GL - 1) / GL;
+    const LV part = X::gen([&](int l) {
+      int t = 0;
+      for (int p = l * run; p < (l + 1) * run && p < np; p++) t += cnt[p];
+      return (Real)t;
+    });
+    LV pb[GL];
+    bcast_all<X>(part, pb);
+    X::each([&](int l) {
+      int o = 0;
+#pragma unroll
+      for (int k = 0; k < GL; k++)
+        if (k < l) o += (int)X::at(pb[k], l);
+      for (int p = l * run; p < (l + 1) * run && p < np; p++) {
+        off[p] = o;
+        o += cnt[p];
+      }
+      if (l == 0) {
+        int tot = 0;
+#pragma unroll
+        for (int k = 0; k < GL; k++) tot += (int)X::at(pb[k], l);
+        if (tot > M.ncmax) {  // only reachable with a user-lowered contact capacity
+          isc[7] |= ST_CONTACT_OVERFLOW;
+          tot = M.ncmax;
+        }
+        isc[0] = tot;  // ncon
+      }
+    });
+    X::sync();
+  }
+  auto con = S.con + E.e * (int64_t)M.ncmax * CW;
+  auto cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
   X::par(M.npair, [&](int p) {
     int n = cnt[p];
     for (int h = 0; h < n && off[p] + h < M.ncmax; h++) {
-      const Real* src = stage + ((int64_t)p * MAXHIT + h) * CW;
-      Real* dst = con + (int64_t)(off[p] + h) * CW;
+      auto src = stage + ((int64_t)p * MAXHIT + h) * CW;
+      auto dst = con + (int64_t)(off[p] + h) * CW;
       for (int k = 0; k < CW; k++) dst[k] = src[k];
-      int* gd = cg + (off[p] + h) * 3;
+      auto gd = cg + (off[p] + h) * 3;
       gd[0] = M.pair_g1[p];
       gd[1] = M.pair_g2[p];
       gd[2] = h;
@@ -379,7 +417,8 @@ PHD void collide(const DModel& M, const DState& S, Env& E) {
 }
 
 // Contact::frame, collision.rs:47-55
-PHD void contact_frame(const Real* c, V3* t1, V3* t2) {
+template <class P>
+PHD void contact_frame(P c, V3* t1, V3* t2) {
   V3 n = ld3(c + 3), tg = ld3(c + 6);
   V3 t = tg - n * dot(n, tg);
   if (norm2(t) > 1e-12) {
@@ -500,6 +539,49 @@ PHD void mass_solve(const DModel& M, Env& E, Real* x) {
     chol_solve_block<X>(M, E, LD(chol), M.nv, M.block_adr[blk], M.block_adr[blk + 1], x);
 }
 
+// factor_mass then mass_solve for the common case (Euler, one dof block of
+// at most 16 dofs) in registers: M + h diag(damping) one row per lane, the
+// register Cholesky, then the two triangular solves. Other models take the
+// general path. Returns false when the matrix is not positive definite.
+template <class X>
+PHD bool factor_solve_mass(const DModel& M, Env& E, Real h, Real* x) {
+  if (M.integrator == 0 && M.nblock == 1 && M.nv <= GL) {
+    using LV = typename X::LV;
+    const int nv = M.nv;
+    const Real* qm = LD(qm);
+    auto damp = M.v_dof_damping + (int64_t)E.var * nv;
+    LV r[GL];
+#pragma unroll
+    for (int k = 0; k < GL; k++)
+      r[k] = X::gen([&](int l) {
+        Real v = qm[clampi(l, nv) * nv + clampi(k, nv)];
+        return (l < nv && k < nv) ? v : 0.0;
+      });
+    // add_implicit_terms (forward.rs:407-431): the diagonal gains h * damping
+    const LV hd = X::gen([&](int l) { return h * damp[clampi(l, nv)]; });
+#pragma unroll
+    for (int k = 0; k < GL; k++) r[k] = X::sel([&](int l) { return l == k; }, r[k] + hd, r[k]);
+    if (!chol_rows<X>(r, nv)) return false;
+    store_lower<X>(r, LD(chol), nv, 0, nv);
+    X::sync();
+    LV lr[GL], lc[GL], lii;
+    load_factor<X>(LD(chol), nv, 0, nv, lr, lc, &lii);
+    LV xv = X::gen([&](int l) {
+      Real v = x[clampi(l, nv)];
+      return l < nv ? v : 0.0;
+    });
+    xv = solve_rows<X>(r, lc, lii, xv, nv);
+    X::each([&](int l) {
+      if (l < nv) x[l] = X::at(xv, l);
+    });
+    X::sync();
+    return true;
+  }
+  if (!factor_mass<X>(M, E, h)) return false;
+  mass_solve<X>(M, E, x);
+  return true;
+}
+
 // ---------------- constraint rows, newton.rs:113-218 ----------------
 // Returns nr; rows are bound to LDS or global storage by their count.
 template <class X>
@@ -507,21 +589,66 @@ PHD int build_rows(const DModel& M, const DState& S, Env& E) {
   const int nv = M.nv;
   int* isc = E.si + M.L.i_iscal;
   int* cact = reinterpret_cast<int*>(LD(cact));
-  const Real* con = S.con + E.e * (int64_t)M.ncmax * CW;
-  const int* cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
+  auto con = S.con + E.e * (int64_t)M.ncmax * CW;
+  auto cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
   const Real* qpos = LD(qpos);
-  // list of touching contacts, then limit and equality counts
+  // Violated limits per joint (bit 0 lower, bit 1 upper), in joint order.
+  // With at most 16 joints each lane owns a joint and the row positions come
+  // from a scan across the lanes; otherwise one lane walks the joints.
+  const bool lane_limits = M.njnt <= GL;
+  using LV = typename X::LV;
+  auto seps_of = [&](int j, Real* s0, Real* s1) {
+    int t = M.jnt_type[j];
+    if (!M.jnt_limited[j] || !(t == J_HINGE || t == J_SLIDE)) return 0;
+    Real q = qpos[M.jnt_qposadr[j]];
+    *s0 = q - M.jnt_range[2 * j];
+    *s1 = M.jnt_range[2 * j + 1] - q;
+    return (*s0 < 0.0 ? 1 : 0) + (*s1 < 0.0 ? 2 : 0);
+  };
+  LV lim_before = X::splat(0.0), lim_flags = X::splat(0.0);
+  if (lane_limits) {
+    lim_flags = X::gen([&](int l) {
+      Real s0 = 0.0, s1 = 0.0;
+      return l < M.njnt ? (Real)seps_of(l, &s0, &s1) : 0.0;
+    });
+    LV fb[GL];
+    bcast_all<X>(lim_flags, fb);
+    lim_before = X::gen([&](int l) {
+      int n = 0;
+#pragma unroll
+      for (int k = 0; k < GL; k++) {
+        int f = (int)X::at(fb[k], l);
+        if (k < l) n += (f & 1) + (f >> 1);
+      }
+      return (Real)n;
+    });
+    X::each([&](int l) {
+      if (l == 0) {
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < GL; k++) {
+          int f = (int)X::at(fb[k], l);
+          n += (f & 1) + (f >> 1);
+        }
+        isc[4] = n;
+      }
+    });
+    X::sync();
+  }
+  // list of touching contacts, then the row count
   X::par(1, [&](int) {
     int ncon = isc[0], na = 0;
     for (int c = 0; c < ncon; c++)
       if (con[c * CW + 9] > 0.0) cact[na++] = c;
     int nl = 0;
-    for (int j = 0; j < M.njnt; j++) {
-      int t = M.jnt_type[j];
-      if (!M.jnt_limited[j] || !(t == J_HINGE || t == J_SLIDE)) continue;
-      Real q = qpos[M.jnt_qposadr[j]];
-      if (q - M.jnt_range[2 * j] < 0.0) nl++;
-      if (M.jnt_range[2 * j + 1] - q < 0.0) nl++;
+    if (lane_limits) {
+      nl = isc[4];
+    } else {
+      for (int j = 0; j < M.njnt; j++) {
+        Real s0, s1;
+        int f = seps_of(j, &s0, &s1);
+        nl += (f & 1) + (f >> 1);
+      }
     }
     isc[2] = na;
     isc[3] = 4 * na + nl + M.neq;
@@ -539,7 +666,7 @@ PHD int build_rows(const DModel& M, const DState& S, Env& E) {
   X::par(na * nv, [&](int t) {
     int k = t / nv, dof = t % nv;
     int c = cact[k];
-    const Real* cc = con + c * CW;
+    auto cc = con + c * CW;
     int g1 = cg[3 * c], g2 = cg[3 * c + 1];
     int b1 = M.geom_body[g1], b2 = M.geom_body[g2];
     Real mu = rmax(M.v_friction[(int64_t)E.var * M.ngeom + g1], M.v_friction[(int64_t)E.var * M.ngeom + g2]);
@@ -567,7 +694,7 @@ PHD int build_rows(const DModel& M, const DState& S, Env& E) {
     int g1 = cg[3 * c], g2 = cg[3 * c + 1];
     int b1 = M.geom_body[g1], b2 = M.geom_body[g2];
     Real mu = rmax(M.v_friction[(int64_t)E.var * M.ngeom + g1], M.v_friction[(int64_t)E.var * M.ngeom + g2]);
-    const Real* iw = M.v_body_invweight + (int64_t)E.var * M.nbody * 2;
+    auto iw = M.v_body_invweight + (int64_t)E.var * M.nbody * 2;
     Real tran = iw[2 * b1] + iw[2 * b2];
     Real dpyr = 2.0 * mu * mu * (1.0 + mu * mu) * tran / M.impratio;
     for (int r = 0; r < 4; r++) {
@@ -575,29 +702,44 @@ PHD int build_rows(const DModel& M, const DState& S, Env& E) {
       E.rdiag[4 * k + r] = dpyr;
     }
   });
-  // joint limits then equalities, in joint order (serial: few rows)
+  // joint limit rows, in joint order (lower then upper), then equalities
+  auto iw = M.v_dof_invweight + (int64_t)E.var * nv;
+  auto limit_rows = [&](int j, int f, int r) {
+    Real seps[2];
+    seps_of(j, &seps[0], &seps[1]);
+    int da = M.jnt_dofadr[j];
+    for (int s = 0; s < 2; s++) {
+      if ((f >> s) & 1) {
+        for (int k = 0; k < nv; k++) J[r * nv + k] = 0.0;
+        J[r * nv + da] = s == 0 ? 1.0 : -1.0;
+        E.rpos[r] = seps[s];
+        E.rdiag[r] = iw[da];
+        r++;
+      }
+    }
+    return r;
+  };
+  if (lane_limits) {
+    X::each([&](int l) {
+      int f = (int)X::at(lim_flags, l);
+      if (l < M.njnt && f) limit_rows(l, f, 4 * na + (int)X::at(lim_before, l));
+    });
+    X::sync();
+  }
   X::par(1, [&](int) {
     int r = 4 * na;
-    const Real* iw = M.v_dof_invweight + (int64_t)E.var * nv;
-    for (int j = 0; j < M.njnt; j++) {
-      int t = M.jnt_type[j];
-      if (!M.jnt_limited[j] || !(t == J_HINGE || t == J_SLIDE)) continue;
-      Real q = qpos[M.jnt_qposadr[j]];
-      int da = M.jnt_dofadr[j];
-      Real seps[2] = {q - M.jnt_range[2 * j], M.jnt_range[2 * j + 1] - q};
-      for (int s = 0; s < 2; s++) {
-        if (seps[s] < 0.0) {
-          for (int k = 0; k < nv; k++) J[r * nv + k] = 0.0;
-          J[r * nv + da] = s == 0 ? 1.0 : -1.0;
-          E.rpos[r] = seps[s];
-          E.rdiag[r] = iw[da];
-          r++;
-        }
+    if (lane_limits) {
+      r += isc[4];
+    } else {
+      for (int j = 0; j < M.njnt; j++) {
+        Real s0, s1;
+        int f = seps_of(j, &s0, &s1);
+        if (f) r = limit_rows(j, f, r);
       }
     }
     for (int e = 0; e < M.neq; e++) {
       int j1 = M.eq_j1[e], j2 = M.eq_j2[e];
-      const Real* c = M.eq_poly + 5 * e;
+      auto c = M.eq_poly + 5 * e;
       Real q1 = qpos[M.jnt_qposadr[j1]] - M.qpos0[M.jnt_qposadr[j1]];
       Real poly = c[0], dpoly = 0.0;
       if (j2 >= 0) {
@@ -621,8 +763,9 @@ PHD int build_rows(const DModel& M, const DState& S, Env& E) {
   const int n_ineq = nr - M.neq;
   X::par(nr, [&](int r) {
     bool bil = r >= n_ineq;
-    const Real* solref = bil ? M.eq_solref + 2 * (r - n_ineq) : M.solref;
-    const Real* solimp = bil ? M.eq_solimp + 5 * (r - n_ineq) : M.solimp;
+    Real solref[2], solimp[5];
+    for (int i = 0; i < 2; i++) solref[i] = bil ? M.eq_solref[2 * (r - n_ineq) + i] : M.solref[i];
+    for (int i = 0; i < 5; i++) solimp[i] = bil ? M.eq_solimp[5 * (r - n_ineq) + i] : M.solimp[i];
     Real tc = rmax(solref[0], 2.0 * M.timestep), dr = solref[1];
     Real dmax = solimp[1];
     Real b = 2.0 / (dmax * tc);
@@ -658,11 +801,20 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
   using LV = typename X::LV;
   const int nv = M.nv;
   const int n_ineq = nr - M.neq;
+  // rows are in LDS on this path (nr <= nrl): address them through the LDS
+  // layout so every access is an LDS access
   const Real* qm = LD(qm);
-  const Real* J = E.J;
+  const Real* J = LD(J);
+  const Real* rdd = LD(dd);
   Real* Lf = LD(chol);
-  const LV aref = X::gen([&](int l) { return l < nr ? E.aref[l] : 0.0; });
-  const LV dd = X::gen([&](int l) { return l < nr ? E.dd[l] : 0.0; });
+  const LV aref = X::gen([&](int l) {
+    Real v = LD(aref)[clampi(l, nr)];
+    return l < nr ? v : 0.0;
+  });
+  const LV dd = X::gen([&](int l) {
+    Real v = rdd[clampi(l, nr)];
+    return l < nr ? v : 0.0;
+  });
   LV ddb[GL];
   bcast_all<X>(dd, ddb);
   // active(v): bilateral rows always, inequality rows when v < 0
@@ -674,7 +826,11 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
     LV xb[GL], mr[GL];
     bcast_all<X>(x, xb);
 #pragma unroll
-    for (int k = 0; k < GL; k++) mr[k] = X::gen([&](int l) { return (l < nv && k < nv) ? qm[l * nv + k] : 0.0; });
+    for (int k = 0; k < GL; k++)
+      mr[k] = X::gen([&](int l) {
+        Real v = qm[clampi(l, nv) * nv + clampi(k, nv)];
+        return (l < nv && k < nv) ? v : 0.0;
+      });
     return lsdot_rt<X>(mr, xb, nv) + 0.0 * x;
   };
   // J x, one row per lane
@@ -682,11 +838,20 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
     LV xb[GL], jr[GL];
     bcast_all<X>(x, xb);
 #pragma unroll
-    for (int k = 0; k < GL; k++) jr[k] = X::gen([&](int l) { return (l < nr && k < nv) ? J[l * nv + k] : 0.0; });
+    for (int k = 0; k < GL; k++)
+      jr[k] = X::gen([&](int l) {
+        Real v = J[clampi(l, nr) * nv + clampi(k, nv)];
+        return (l < nr && k < nv) ? v : 0.0;
+      });
     return lsdot_rt<X>(jr, xb, nv);
   };
   // column r of J^T: lane k gets J[r][k]
-  auto jrow = [&](int r) { return X::gen([&](int l) { return l < nv ? J[r * nv + l] : 0.0; }); };
+  auto jrow = [&](int r) {
+    return X::gen([&](int l) {
+      Real v = J[r * nv + clampi(l, nv)];
+      return l < nv ? v : 0.0;
+    });
+  };
   // sum over active rows of f_r J_r (a dof vector), rows in order, from `init`
   auto add_rows = [&](LV g, const LV& f, uint32_t act) {
     LV fb[GL];
@@ -711,10 +876,16 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
     return c;
   };
 
-  const LV a0 = X::gen([&](int l) { return l < nv ? LD(a)[l] : 0.0; });
+  const LV a0 = X::gen([&](int l) {
+    Real v = LD(a)[clampi(l, nv)];
+    return l < nv ? v : 0.0;
+  });
   LV a = a0;
   {
-    const LV warm = X::gen([&](int l) { return l < nv ? LD(qacc)[l] : 0.0; });
+    const LV warm = X::gen([&](int l) {
+      Real v = LD(qacc)[clampi(l, nv)];
+      return l < nv ? v : 0.0;
+    });
     Real c0 = cost(a0, a0), c1 = cost(warm, a0);
     if (c1 < c0) a = warm;
   }
@@ -737,13 +908,16 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
     LV h[GL];
 #pragma unroll
     for (int j = 0; j < GL; j++) {
-      h[j] = X::gen([&](int l) { return (l < nv && j < nv) ? qm[l * nv + j] : 0.0; });
+      h[j] = X::gen([&](int l) {
+        Real v = qm[clampi(l, nv) * nv + clampi(j, nv)];
+        return (l < nv && j < nv) ? v : 0.0;
+      });
       h[j] = X::sel([&](int l) { return l == j; }, h[j] + 0.0, h[j]);
     }
     for (int r = 0; r < nr; r++) {
       if (!((act >> r) & 1u)) continue;
       const LV ji = jrow(r);
-      const LV di = E.dd[r] * ji;
+      const LV di = rdd[r] * ji;
 #pragma unroll
       for (int j = 0; j < GL; j++)
         if (j < nv) h[j] = X::sel([&](int l) { return X::at(ji, l) != 0.0; }, h[j] + di * J[r * nv + j], h[j]);
@@ -753,7 +927,7 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
     X::sync();
     LV lr[GL], lc[GL], lii;
     load_factor<X>(Lf, nv, 0, nv, lr, lc, &lii);
-    const LV p = solve_rows<X>(lr, lc, lii, -grad, nv);
+    const LV p = solve_rows<X>(h, lc, lii, -grad, nv);
     // exact line search along p (newton.rs:310-351)
     const LV mp = mul_m(p);
     const Real mpp = X::uni(lsum_lanes<X>(p * mp, nv, -0.0));
@@ -1015,11 +1189,10 @@ PHD bool step_env(const DModel& M, const DState& S, Env& E) {
     isc[1] = 0;
     isc[5] = 0;
   });
-  if (!factor_mass<X>(M, E, 0.0)) {
+  if (!factor_solve_mass<X>(M, E, 0.0, a)) {
     X::par(1, [&](int) { isc[7] |= ST_NOTPD; });
     return false;
   }
-  mass_solve<X>(M, E, a);
   X::par(nv, [&](int k) { LD(qfrc_c)[k] = 0.0; });
   int nr = build_rows<X>(M, S, E);
   if (nr < 0) {
@@ -1034,12 +1207,11 @@ PHD bool step_env(const DModel& M, const DState& S, Env& E) {
   const bool damped = M.v_damped[E.var] != 0;
   if (damped) {
     X::par(1, [&](int) { isc[1] = 0; });
-    if (!factor_mass<X>(M, E, h)) {
+    X::par(nv, [&](int k) { acc[k] = smooth[k] + LD(qfrc_c)[k]; });
+    if (!factor_solve_mass<X>(M, E, h, acc)) {
       X::par(1, [&](int) { isc[7] |= ST_NOTPD; });
       return false;
     }
-    X::par(nv, [&](int k) { acc[k] = smooth[k] + LD(qfrc_c)[k]; });
-    mass_solve<X>(M, E, acc);
   } else {
     X::par(nv, [&](int k) { acc[k] = a[k]; });
   }

@@ -17,6 +17,10 @@ namespace phk {
 
 constexpr int GL = 16;  // lanes per environment group
 
+// Clamp an index into [0, n): lets a lane load unconditionally (every address
+// in bounds, no branch around the load) and select the value afterwards.
+PHD int clampi(int i, int n) { return i < n ? i : n - 1; }
+
 // compile-time loop: f(integral_constant<int, I>) for I in [B, E) while f returns true
 template <int B, int E, class F>
 PHD void sfor(F&& f) {
@@ -293,7 +297,10 @@ PHD bool chol_lanes(Real* A, int nv, int s0, int n) {
   LV r[GL];
 #pragma unroll
   for (int k = 0; k < GL; k++)
-    r[k] = X::gen([&](int l) { return (l < n && k < n) ? A[(s0 + l) * nv + s0 + k] : 0.0; });
+    r[k] = X::gen([&](int l) {
+      Real v = A[(s0 + clampi(l, n)) * nv + s0 + clampi(k, n)];
+      return (l < n && k < n) ? v : 0.0;
+    });
   if (!chol_rows<X>(r, n)) return false;
   store_lower<X>(r, A, nv, s0, n);
   return true;
@@ -330,10 +337,20 @@ template <class X, class LV>
 PHD void load_factor(const Real* L, int nv, int s0, int n, LV* r, LV* c, LV* lii) {
 #pragma unroll
   for (int k = 0; k < GL; k++) {
-    r[k] = X::gen([&](int l) { return (l < n && k < l) ? L[(s0 + l) * nv + s0 + k] : 0.0; });
-    c[k] = X::gen([&](int l) { return (k < n && k > l) ? L[(s0 + k) * nv + s0 + l] : 0.0; });
+    r[k] = X::gen([&](int l) {
+      Real v = L[(s0 + clampi(l, n)) * nv + s0 + clampi(k, n)];
+      return (l < n && k < l) ? v : 0.0;
+    });
+    c[k] = X::gen([&](int l) {
+      Real v = L[(s0 + clampi(k, n)) * nv + s0 + clampi(l, n)];
+      return (k < n && k > l) ? v : 0.0;
+    });
   }
-  *lii = X::gen([&](int l) { return l < n ? L[(s0 + l) * nv + s0 + l] : 1.0; });
+  *lii = X::gen([&](int l) {
+    int m = clampi(l, n);
+    Real v = L[(s0 + m) * nv + s0 + m];
+    return l < n ? v : 1.0;
+  });
 }
 
 template <class X>
@@ -341,7 +358,10 @@ PHD void solve_lanes(const Real* L, int nv, int s0, int n, Real* x) {
   using LV = typename X::LV;
   LV r[GL], c[GL], lii;
   load_factor<X>(L, nv, s0, n, r, c, &lii);
-  LV xv = X::gen([&](int l) { return l < n ? x[s0 + l] : 0.0; });
+  LV xv = X::gen([&](int l) {
+    Real v = x[s0 + clampi(l, n)];
+    return l < n ? v : 0.0;
+  });
   xv = solve_rows<X>(r, c, lii, xv, n);
   X::each([&](int l) {
     if (l < n) x[s0 + l] = X::at(xv, l);

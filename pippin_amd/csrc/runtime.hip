@@ -228,7 +228,7 @@ struct ph_sim {
 extern "C" const char* pippin_hip_last_error(void) { return g_err.c_str(); }
 
 template <class T>
-static hipError_t up(DBuf<T>& b, const std::vector<T>& v, const T** out) {
+static hipError_t up(DBuf<T>& b, const std::vector<T>& v, gp<const T>* out) {
   hipError_t e = b.upload(v);
   *out = b.p;
   return e;
@@ -376,8 +376,8 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   }
 
   int ii = 0, fi = 0;
-  auto ui = [&](const std::vector<int32_t>& v, const int** dst) -> hipError_t { return up(s->d_i[ii++], v, dst); };
-  auto uf = [&](const std::vector<double>& v, const Real** dst) -> hipError_t { return up(s->d_f[fi++], v, dst); };
+  auto ui = [&](const std::vector<int32_t>& v, gp<const int>* dst) -> hipError_t { return up(s->d_i[ii++], v, dst); };
+  auto uf = [&](const std::vector<double>& v, gp<const Real>* dst) -> hipError_t { return up(s->d_f[fi++], v, dst); };
   HIPCHK(ui(hm.body_parent, &M.body_parent));
   HIPCHK(ui(hm.body_jntadr, &M.body_jntadr));
   HIPCHK(ui(hm.body_jntnum, &M.body_jntnum));
@@ -386,6 +386,10 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   HIPCHK(ui(hm.body_lastdof, &M.body_lastdof));
   HIPCHK(ui(hm.level_adr, &M.level_adr));
   HIPCHK(ui(hm.level_body, &M.level_body));
+  HIPCHK(ui(hm.acc_level_adr, &M.acc_level_adr));
+  HIPCHK(ui(hm.acc_parent, &M.acc_parent));
+  HIPCHK(ui(hm.acc_child_adr, &M.acc_child_adr));
+  HIPCHK(ui(hm.acc_child, &M.acc_child));
   HIPCHK(ui(hm.jnt_type, &M.jnt_type));
   HIPCHK(ui(hm.jnt_qposadr, &M.jnt_qposadr));
   HIPCHK(ui(hm.jnt_dofadr, &M.jnt_dofadr));

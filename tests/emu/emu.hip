@@ -69,6 +69,10 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   M.body_quat = hm.body_quat.data();
   M.level_adr = hm.level_adr.data();
   M.level_body = hm.level_body.data();
+  M.acc_level_adr = hm.acc_level_adr.data();
+  M.acc_parent = hm.acc_parent.data();
+  M.acc_child_adr = hm.acc_child_adr.data();
+  M.acc_child = hm.acc_child.data();
   M.jnt_type = hm.jnt_type.data();
   M.jnt_qposadr = hm.jnt_qposadr.data();
   M.jnt_dofadr = hm.jnt_dofadr.data();
