@@ -24,14 +24,35 @@ FLAGS = [
 ]
 
 
+# Models whose kernels are also built with compile-time sizes and LDS layout
+# (struct name, MJCF file); any other model runs the generic kernel.
+SPECIALISED = [("SpecAnt", HERE / "assets" / "ant.xml")]
+
+
+def generate_specs(bdir: Path, verbose: bool = False) -> None:
+    """Build the host-only generator and write _build/spec_models.hpp."""
+    gen = bdir / "spec_gen"
+    srcs = [CSRC / "spec_gen.cpp", CSRC / "host_model.cpp", CSRC / "mjcf.cpp"]
+    cmd = [HIPCC, "-O1", "-std=c++17", "-x", "hip", f"--offload-arch={ARCH}", *map(str, srcs), "-o", str(gen)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stderr=subprocess.DEVNULL if not verbose else None)
+    args = [str(gen)]
+    for name, path in SPECIALISED:
+        args += [name, str(path)]
+    out = subprocess.run(args, check=True, capture_output=True, text=True).stdout
+    (bdir / "spec_models.hpp").write_text(out)
+
+
 def build(verbose: bool = False) -> Path:
     objs = []
     bdir = HERE / "_build"
     bdir.mkdir(exist_ok=True)
+    generate_specs(bdir, verbose)
     for src in SOURCES:
         obj = bdir / (src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC, *FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [HIPCC, *FLAGS, *lang, "-I", str(bdir), "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -3,6 +3,9 @@
 // sizes, passed to kernels by value.
 #pragma once
 #include <stdint.h>
+#include <string.h>
+
+#include <cstring>
 
 #include "dmath.hpp"
 
@@ -80,6 +83,8 @@ struct Layout {
   int nint;
   int bytes;  // per env, 16-byte aligned
 };
+
+inline bool layout_equal(const Layout& a, const Layout& b) { return std::memcmp(&a, &b, sizeof(Layout)) == 0; }
 
 struct DModel {
   int nq, nv, nu, nbody, njnt, ngeom, ncam, neq, npair, nlevel, nblock, nacc;

@@ -28,8 +28,8 @@ struct Env {
 
 #define LD(name) (E.s + M.L.name)
 
-template <class X>
-PHD void rows_bind(const DModel& M, const DState& S, Env& E, int nr) {
+template <class X, class MT>
+PHD void rows_bind(const MT& M, const DState& S, Env& E, int nr) {
   if (nr <= M.nrl) {
     E.J = LD(J);
     E.aref = LD(aref);
@@ -57,8 +57,8 @@ PHD void rows_bind(const DModel& M, const DState& S, Env& E, int nr) {
 }
 
 // ---------------- kinematics, forward.rs:14-90 ----------------
-template <class X>
-PHD void fk_body(const DModel& M, Env& E, int b) {
+template <class X, class MT>
+PHD void fk_body(const MT& M, Env& E, int b) {
   Real* xpos = LD(xpos);
   Real* xquat = LD(xquat);
   Real* xmat = LD(xmat);
@@ -118,8 +118,8 @@ PHD void fk_body(const DModel& M, Env& E, int b) {
 // Body centre of mass and composite-ready spatial inertia (the tail of each
 // body's step in forward.rs:14-90). They depend only on the body's own pose,
 // not on the tree order, so all bodies are done at once after the levels.
-template <class X>
-PHD void body_inertia(const DModel& M, Env& E, int b) {
+template <class X, class MT>
+PHD void body_inertia(const MT& M, Env& E, int b) {
   V3 pos = ld3(LD(xpos) + 3 * b);
   M3 xm = ldm(LD(xmat) + 9 * b);
   V3 xipos = pos + mul_vec(xm, ld3(M.v_body_ipos + ((int64_t)E.var * M.nbody + b) * 3));
@@ -135,8 +135,8 @@ PHD void body_inertia(const DModel& M, Env& E, int b) {
   stm(ci + 4, io);
 }
 
-template <class X>
-PHD void kinematics(const DModel& M, Env& E) {
+template <class X, class MT>
+PHD void kinematics(const MT& M, Env& E) {
   // hinge rotations do not depend on the parent's pose: evaluate their
   // sin/cos for all joints at once, off the tree's serial path
   X::par(M.njnt, [&](int j) {
@@ -163,8 +163,8 @@ PHD void kinematics(const DModel& M, Env& E) {
 // Geom world poses, the last loop of forward.rs:14-90. Evaluated where they
 // are consumed (collision, pose export) so that they need no LDS of their own
 // across the dynamics stages.
-template <class X>
-PHD void geom_poses(const DModel& M, Env& E) {
+template <class X, class MT>
+PHD void geom_poses(const MT& M, Env& E) {
   X::par(M.ngeom, [&](int g) {
     int b = M.geom_body[g];
     M3 xm = ldm(LD(xmat) + 9 * b);
@@ -175,8 +175,8 @@ PHD void geom_poses(const DModel& M, Env& E) {
 
 // x[parent(b)] += x[b] for b from the last body down to 1 (w components per
 // body), level by level: lanes take (parent, component) pairs of one level.
-template <class X>
-PHD void leaf_to_root(const DModel& M, Real* x, int w) {
+template <class X, class MT>
+PHD void leaf_to_root(const MT& M, Real* x, int w) {
   for (int L = 0; L < M.nacc; L++) {
     int g0 = M.acc_level_adr[L], ng = M.acc_level_adr[L + 1] - g0;
     X::par(ng * w, [&](int t) {
@@ -190,8 +190,8 @@ PHD void leaf_to_root(const DModel& M, Real* x, int w) {
 }
 
 // ---------------- CRBA, forward.rs:93-112 ----------------
-template <class X>
-PHD void crba(const DModel& M, Env& E) {
+template <class X, class MT>
+PHD void crba(const MT& M, Env& E) {
   const int nb = M.nbody, nv = M.nv;
   Real* crb = LD(crb);
   const Real* ci = LD(cinert);
@@ -222,8 +222,8 @@ PHD void crba(const DModel& M, Env& E) {
 // acceleration sum in dof order, exactly where the reference's second loop
 // would read it, so it never needs storage. Then the backward force sum and
 // qfrc_bias (forward.rs:166-171).
-template <class X>
-PHD void velocity_rne(const DModel& M, Env& E) {
+template <class X, class MT>
+PHD void velocity_rne(const MT& M, Env& E) {
   Real* cvel = LD(cvel);
   Real* cacc = LD(cacc);
   Real* cfrc = LD(cfrc);
@@ -271,8 +271,8 @@ PHD void velocity_rne(const DModel& M, Env& E) {
 }
 
 // ---------------- passive + actuation, forward.rs:173-211 ----------------
-template <class X>
-PHD void passive_actuation(const DModel& M, Env& E) {
+template <class X, class MT>
+PHD void passive_actuation(const MT& M, Env& E) {
   const Real* qpos = LD(qpos);
   const Real* qvel = LD(qvel);
   X::par(M.nv, [&](int k) {
@@ -315,8 +315,8 @@ PHD void passive_actuation(const DModel& M, Env& E) {
 }
 
 // ---------------- collision, forward.rs:214-260 ----------------
-template <class X>
-PHD void collide(const DModel& M, const DState& S, Env& E) {
+template <class X, class MT>
+PHD void collide(const MT& M, const DState& S, Env& E) {
   const int ng = M.ngeom;
   geom_poses<X>(M, E);
   Real* aabb = LD(aabb);
@@ -454,8 +454,8 @@ PHD Real impedance(const Real* solimp, Real pos) {
 // Factor the diagonal block [s0, e) of the nv x nv row-major matrix A in place.
 // Column j: the diagonal (one lane), then the entries below it (one lane per
 // row, each the reference's 4-accumulator dot). Returns false if not PD.
-template <class X>
-PHD bool chol_block(const DModel& M, Env& E, Real* A, int nv, int s0, int e) {
+template <class X, class MT>
+PHD bool chol_block(const MT& M, Env& E, Real* A, int nv, int s0, int e) {
   if (e - s0 <= GL) {
     bool ok = chol_lanes<X>(A, nv, s0, e - s0);
     X::sync();
@@ -485,8 +485,8 @@ PHD bool chol_block(const DModel& M, Env& E, Real* A, int nv, int s0, int e) {
 }
 
 // Forward then backward substitution on block [s0, e); x is an LDS vector.
-template <class X>
-PHD void chol_solve_block(const DModel& M, Env& E, const Real* L, int nv, int s0, int e, Real* x) {
+template <class X, class MT>
+PHD void chol_solve_block(const MT& M, Env& E, const Real* L, int nv, int s0, int e, Real* x) {
   (void)M;
   (void)E;
   if (e - s0 <= GL) {
@@ -505,8 +505,8 @@ PHD void chol_solve_block(const DModel& M, Env& E, const Real* L, int nv, int s0
   });
 }
 
-template <class X>
-PHD bool factor_mass(const DModel& M, Env& E, Real h) {
+template <class X, class MT>
+PHD bool factor_mass(const MT& M, Env& E, Real h) {
   const int nv = M.nv;
   Real* chol = LD(chol);
   const Real* qm = LD(qm);
@@ -533,8 +533,8 @@ PHD bool factor_mass(const DModel& M, Env& E, Real h) {
   return true;
 }
 
-template <class X>
-PHD void mass_solve(const DModel& M, Env& E, Real* x) {
+template <class X, class MT>
+PHD void mass_solve(const MT& M, Env& E, Real* x) {
   for (int blk = 0; blk < M.nblock; blk++)
     chol_solve_block<X>(M, E, LD(chol), M.nv, M.block_adr[blk], M.block_adr[blk + 1], x);
 }
@@ -543,8 +543,8 @@ PHD void mass_solve(const DModel& M, Env& E, Real* x) {
 // at most 16 dofs) in registers: M + h diag(damping) one row per lane, the
 // register Cholesky, then the two triangular solves. Other models take the
 // general path. Returns false when the matrix is not positive definite.
-template <class X>
-PHD bool factor_solve_mass(const DModel& M, Env& E, Real h, Real* x) {
+template <class X, class MT>
+PHD bool factor_solve_mass(const MT& M, Env& E, Real h, Real* x) {
   if (M.integrator == 0 && M.nblock == 1 && M.nv <= GL) {
     using LV = typename X::LV;
     const int nv = M.nv;
@@ -584,8 +584,8 @@ PHD bool factor_solve_mass(const DModel& M, Env& E, Real h, Real* x) {
 
 // ---------------- constraint rows, newton.rs:113-218 ----------------
 // Returns nr; rows are bound to LDS or global storage by their count.
-template <class X>
-PHD int build_rows(const DModel& M, const DState& S, Env& E) {
+template <class X, class MT>
+PHD int build_rows(const MT& M, const DState& S, Env& E) {
   const int nv = M.nv;
   int* isc = E.si + M.L.i_iscal;
   int* cact = reinterpret_cast<int*>(LD(cact));
@@ -783,8 +783,8 @@ PHD int build_rows(const DModel& M, const DState& S, Env& E) {
 }
 
 // ---------------- Newton, newton.rs:224-371 (single island) ----------------
-template <class X>
-PHD void mul_m(const DModel& M, Env& E, const Real* x, Real* y) {
+template <class X, class MT>
+PHD void mul_m(const MT& M, Env& E, const Real* x, Real* y) {
   const int nv = M.nv;
   const Real* qm = LD(qm);
   X::par(nv, [&](int i) { y[i] = sdot_n(qm + i * nv, x, nv) + 0.0 * x[i]; });
@@ -796,8 +796,8 @@ PHD void mul_m(const DModel& M, Env& E, const Real* x, Real* y) {
 // in LDS and are read with independent loads; the Hessian is assembled one row
 // per lane directly into the register Cholesky. Every sum keeps the
 // reference's order, so the result equals newton_dense's bit for bit.
-template <class X>
-PHD int newton_lanes(const DModel& M, Env& E, int nr) {
+template <class X, class MT>
+PHD int newton_lanes(const MT& M, Env& E, int nr) {
   using LV = typename X::LV;
   const int nv = M.nv;
   const int n_ineq = nr - M.neq;
@@ -853,12 +853,18 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
     });
   };
   // sum over active rows of f_r J_r (a dof vector), rows in order, from `init`
+  // (act only has bits of rows < nr; the updates are selects, not branches)
   auto add_rows = [&](LV g, const LV& f, uint32_t act) {
     LV fb[GL];
     bcast_all<X>(f, fb);
 #pragma unroll
-    for (int r = 0; r < GL; r++)
-      if (r < nr && ((act >> r) & 1u)) g = g + fb[r] * jrow(r);
+    for (int r = 0; r < GL; r++) {
+      const LV jr = X::gen([&](int l) {
+        Real v = J[clampi(r, nr) * nv + clampi(l, nv)];
+        return l < nv ? v : 0.0;
+      });
+      g = X::sel([&](int) { return ((act >> r) & 1u) != 0; }, g + fb[r] * jr, g);
+    }
     return g;
   };
   auto cost = [&](const LV& x, const LV& a0v) {
@@ -871,8 +877,10 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
     LV tb[GL];
     bcast_all<X>(term, tb);
 #pragma unroll
-    for (int r = 0; r < GL; r++)
-      if (r < nr && ((on >> r) & 1u)) c += X::uni(tb[r]);
+    for (int r = 0; r < GL; r++) {
+      Real cr = c + X::uni(tb[r]);
+      c = ((on >> r) & 1u) ? cr : c;
+    }
     return c;
   };
 
@@ -915,12 +923,14 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
       h[j] = X::sel([&](int l) { return l == j; }, h[j] + 0.0, h[j]);
     }
     for (int r = 0; r < nr; r++) {
-      if (!((act >> r) & 1u)) continue;
+      const bool on = ((act >> r) & 1u) != 0;
       const LV ji = jrow(r);
       const LV di = rdd[r] * ji;
 #pragma unroll
-      for (int j = 0; j < GL; j++)
-        if (j < nv) h[j] = X::sel([&](int l) { return X::at(ji, l) != 0.0; }, h[j] + di * J[r * nv + j], h[j]);
+      for (int j = 0; j < GL; j++) {
+        Real jrj = J[r * nv + clampi(j, nv)];
+        h[j] = X::sel([&](int l) { return on && X::at(ji, l) != 0.0; }, h[j] + di * jrj, h[j]);
+      }
     }
     if (!chol_rows<X>(h, nv)) break;
     store_lower<X>(h, Lf, nv, 0, nv);
@@ -939,16 +949,15 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
     auto deriv = [&](Real al, Real* d1o, Real* d2o) {
       Real d1 = al * mpp + mpg, d2 = mpp;
 #pragma unroll
-      for (int r = 0; r < GL; r++)
-        if (r < nr) {
-          Real jpr = X::uni(jpb[r]);
-          Real v = X::uni(jarb[r]) + al * jpr;
-          if (r >= n_ineq || v < 0.0) {
-            Real dq = X::uni(ddb[r]) * jpr;
-            d1 += dq * v;
-            d2 += dq * jpr;
-          }
-        }
+      for (int r = 0; r < GL; r++) {
+        Real jpr = X::uni(jpb[r]);
+        Real v = X::uni(jarb[r]) + al * jpr;
+        Real dq = X::uni(ddb[r]) * jpr;
+        bool on = r < nr && (r >= n_ineq || v < 0.0);
+        Real e1 = d1 + dq * v, e2 = d2 + dq * jpr;
+        d1 = on ? e1 : d1;
+        d2 = on ? e2 : d2;
+      }
       *d1o = d1;
       *d2o = d2;
     };
@@ -983,8 +992,8 @@ PHD int newton_lanes(const DModel& M, Env& E, int nr) {
   return iters;
 }
 
-template <class X>
-PHD int newton_dense(const DModel& M, Env& E, int nr) {
+template <class X, class MT>
+PHD int newton_dense(const MT& M, Env& E, int nr) {
   if (M.nv <= GL && nr <= GL && nr <= M.nrl) return newton_lanes<X>(M, E, nr);
   const int nv = M.nv;
   Real* a = LD(a);
@@ -1145,8 +1154,8 @@ PHD int newton_dense(const DModel& M, Env& E, int nr) {
 }
 
 // ---------------- integrate_pos, forward.rs:459-480 ----------------
-template <class X>
-PHD void integrate_pos(const DModel& M, Env& E, const Real* v, Real h) {
+template <class X, class MT>
+PHD void integrate_pos(const MT& M, Env& E, const Real* v, Real h) {
   Real* qpos = LD(qpos);
   X::par(M.njnt, [&](int j) {
     int qa = M.jnt_qposadr[j], da = M.jnt_dofadr[j], t = M.jnt_type[j];
@@ -1166,8 +1175,8 @@ PHD void integrate_pos(const DModel& M, Env& E, const Real* v, Real h) {
 
 // ---------------- one step, forward.rs:336-389 (Newton) ----------------
 // Returns false when the env cannot be stepped (status set).
-template <class X>
-PHD bool step_env(const DModel& M, const DState& S, Env& E) {
+template <class X, class MT>
+PHD bool step_env(const MT& M, const DState& S, Env& E) {
   const int nv = M.nv;
   const Real h = M.timestep;
   int* isc = E.si + M.L.i_iscal;
@@ -1230,8 +1239,8 @@ PHD bool step_env(const DModel& M, const DState& S, Env& E) {
 // Inverse inertia at the reset configuration (the caller loads qpos_reset
 // into LDS qpos): dof_invweight (nv) and body_invweight (nbody x 2). Runs the
 // solves on one lane; it is called once per parameter variant, not per step.
-template <class X>
-PHD void set_const_env(const DModel& M, Env& E, Real* dof_iw, Real* body_iw) {
+template <class X, class MT>
+PHD void set_const_env(const MT& M, Env& E, Real* dof_iw, Real* body_iw) {
   const int nv = M.nv, nb = M.nbody;
   kinematics<X>(M, E);
   crba<X>(M, E);
@@ -1306,8 +1315,8 @@ PHD void set_const_env(const DModel& M, Env& E, Real* dof_iw, Real* body_iw) {
 // ---------------- per-env driver (shared by the kernels and the serial policy) ----------------
 enum { IS_NCON = 0, IS_STATUS = 7, IS_ITERS = 8, IS_NROWS = 9 };
 
-template <class X>
-PHD void load_state(const DModel& M, const DState& S, Env& E) {
+template <class X, class MT>
+PHD void load_state(const MT& M, const DState& S, Env& E) {
   Real* s = E.s;
   const int64_t e = E.e;
   X::par(M.nq, [&](int k) { s[M.L.qpos + k] = S.qpos[e * M.nq + k]; });
@@ -1327,8 +1336,8 @@ PHD void load_state(const DModel& M, const DState& S, Env& E) {
   });
 }
 
-template <class X>
-PHD void store_poses(const DModel& M, const DState& S, Env& E) {
+template <class X, class MT>
+PHD void store_poses(const MT& M, const DState& S, Env& E) {
   geom_poses<X>(M, E);
   const Real* s = E.s;
   const int64_t e = E.e;
@@ -1339,8 +1348,8 @@ PHD void store_poses(const DModel& M, const DState& S, Env& E) {
 }
 
 // Load one env, run nstep steps and store it back.
-template <class X>
-PHD void run_env(const DModel& M, const DState& S, Env& E, int64_t nstep, const Real* ctrl_seq, Real* qpos_out,
+template <class X, class MT>
+PHD void run_env(const MT& M, const DState& S, Env& E, int64_t nstep, const Real* ctrl_seq, Real* qpos_out,
                  int64_t rel) {
   Real* lds = E.s;
   load_state<X>(M, S, E);

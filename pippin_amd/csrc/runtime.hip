@@ -17,6 +17,7 @@
 #include "../../include/pippin_hip.h"
 #include "host_model.hpp"
 #include "kbody.hpp"
+#include "spec_models.hpp"
 
 using namespace phk;
 
@@ -40,11 +41,14 @@ using DevX = DevXK<1>;
 // Advance envs [begin, end) by nstep steps, K per workgroup. With ctrl_seq
 // (envs x nstep x nu) the controls change every step and qpos_out
 // (envs x nstep x nq) receives qpos after each step (Batch::rollout, batch.rs:46-59).
-template <int K>
+// MT is DModel (sizes and layout read at run time) or a model-specialised
+// struct derived from it whose sizes and LDS layout are compile-time constants
+// (spec_models.hpp, generated at build time); both read the same buffers.
+template <int K, class MT>
 __global__ void __launch_bounds__(64) k_step(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
                                              int64_t begin, int64_t end, int64_t nstep, const Real* ctrl_seq,
                                              Real* qpos_out) {
-  const DModel& M = *Mp;
+  const MT& M = *static_cast<const MT*>(Mp);
   const DState& S = *Sp;
   extern __shared__ double lds[];
   const int g = (int)threadIdx.x / DevXK<K>::G;
@@ -56,6 +60,11 @@ __global__ void __launch_bounds__(64) k_step(const DModel* __restrict__ Mp, cons
   E.e = begin + rel;
   E.var = M.env_var[E.e];
   run_env<DevXK<K>>(M, S, E, nstep, ctrl_seq, qpos_out, rel);
+}
+
+static const void* step_kernel(int K, int spec) {
+  if (spec == 1 && K == 4) return (const void*)k_step<4, SpecAnt>;
+  return K == 4 ? (const void*)k_step<4, DModel> : K == 2 ? (const void*)k_step<2, DModel> : (const void*)k_step<1, DModel>;
 }
 
 // Kinematics only (poses after set(qpos) / reset), for listed envs or a range.
@@ -191,6 +200,7 @@ struct ph_sim {
   int64_t n = 0;
   int device = 0;
   int envs_per_wg = 1;  // environments packed into one k_step workgroup
+  int spec = 0;         // 1: the model matches SpecAnt (compile-time sizes and layout)
   // Poses are a function of qpos (PippinCpu reruns kinematics after step and
   // set(Qpos), pippin_cpu.rs:107-110, 183-185); they are recomputed for all
   // envs on the first poses() call after qpos changed.
@@ -368,11 +378,16 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
     int k = std::atoi(ev);
     if ((k == 1 || k == 2 || k == 4) && (int64_t)k * M.L.bytes <= lds_max) s->envs_per_wg = k;
   }
+  // model-specialised kernel when the model's sizes and layout match one built
+  // in (PIPPIN_HIP_SPEC=0 forces the generic kernel)
+  s->spec = SpecAnt::matches(M) ? 1 : 0;
+  if (const char* ev = std::getenv("PIPPIN_HIP_SPEC"))
+    if (std::atoi(ev) == 0) s->spec = 0;
   {
     const int k = s->envs_per_wg;
     const int bytes = k * M.L.bytes;
-    const void* fn = k == 4 ? (const void*)k_step<4> : k == 2 ? (const void*)k_step<2> : (const void*)k_step<1>;
-    if (bytes > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    if (bytes > 64 * 1024)
+      HIPCHK(hipFuncSetAttribute(step_kernel(k, s->spec), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   }
 
   int ii = 0, fi = 0;
@@ -568,15 +583,18 @@ static void launch_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, co
   const int K = s->envs_per_wg;
   const unsigned grid = (unsigned)((end - begin + K - 1) / K);
   const size_t lds = (size_t)K * s->M.L.bytes;
-  if (K == 4)
-    hipLaunchKernelGGL(k_step<4>, dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep,
-                       ctrl, qpos_out);
+  if (s->spec == 1 && K == 4)
+    hipLaunchKernelGGL((k_step<4, SpecAnt>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
+                       end, nstep, ctrl, qpos_out);
+  else if (K == 4)
+    hipLaunchKernelGGL((k_step<4, DModel>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
+                       end, nstep, ctrl, qpos_out);
   else if (K == 2)
-    hipLaunchKernelGGL(k_step<2>, dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep,
-                       ctrl, qpos_out);
+    hipLaunchKernelGGL((k_step<2, DModel>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
+                       end, nstep, ctrl, qpos_out);
   else
-    hipLaunchKernelGGL(k_step<1>, dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep,
-                       ctrl, qpos_out);
+    hipLaunchKernelGGL((k_step<1, DModel>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
+                       end, nstep, ctrl, qpos_out);
 }
 
 extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep) {

@@ -126,3 +126,28 @@ def test_primitive_pairs_match(name):
         co, cg = o.contacts(k), g.contacts(k)
         for f in ("geom1", "geom2", "feature", "depth", "pos", "normal", "tangent"):
             assert np.array_equal(co[f], cg[f]), f
+
+
+def test_specialised_kernel_equals_generic(assets, monkeypatch):
+    """Ant runs a kernel built with compile-time sizes and layout; it must give
+    the generic kernel's results bit for bit (and both the oracle's)."""
+    arr = mjcf.load_file(assets / "ant.xml")
+    n = 96
+    rng = np.random.default_rng(5)
+    sims = []
+    for spec in ("1", "0"):
+        monkeypatch.setenv("PIPPIN_HIP_SPEC", spec)
+        sims.append(HipPhysics(arr, n))
+    q = sims[0].get("qpos") + rng.normal(size=(n, 15)) * 0.05
+    q[:, 3:7] /= np.linalg.norm(q[:, 3:7], axis=1, keepdims=True)
+    for s in sims:
+        s.set("qpos", 0, n, q)
+    say("specialised vs generic kernel, ant", n)
+    for t in range(60):
+        c = rng.uniform(-1, 1, size=(n, 8))
+        for s in sims:
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 2)
+    assert np.array_equal(sims[0].get("qpos"), sims[1].get("qpos"))
+    assert np.array_equal(sims[0].get("qvel"), sims[1].get("qvel"))
+    assert np.array_equal(sims[0].solver_iterations(), sims[1].solver_iterations())
