@@ -35,6 +35,10 @@ FP64_PEAK_TFLOPS = 78.6
 # reference FP64 ops per env-step on this workload, used only with --no-cpu
 OPS_PER_ENV_STEP_DEFAULT = 25000.0
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_step launch of the default workload (16384 envs x 4 substeps),
+# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a known-size copy
+# (profiles/k_step_ant16k.md); other workloads report null.
+TRAFFIC_BYTES_PER_LAUNCH = 33.8e6
 
 
 def log(*a):
@@ -198,10 +202,12 @@ def main():
                                   "p99": float(np.percentile(iters, 99))},
         "contacts_per_env": {"mean": float(ncon.mean()), "max": int(ncon.max())},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH if (n, sub) == (16384, 4) else None,
                      "note": f"FP64 VALU work (MI355X FP64 peak, vector = matrix); {ops:.0f} reference FP64 ops "
                              f"per env-step; step kernel {kernel_s * 1e3:.3f} ms per launch "
-                             f"({n} envs x {sub} steps)"},
+                             f"({n} envs x {sub} steps); traffic = HBM bytes per launch (rocprof FETCH_SIZE + "
+                             f"WRITE_SIZE, copy-calibrated), far below the compute time at 8 TB/s"},
     }
     if cb is not None:
         line["cpu_baseline"] = cb
