@@ -105,6 +105,8 @@ struct DModel {
   gp<const uint64_t> dof_anc;  // strict ancestors of each dof
   gp<const Real> dof_armature;
   gp<const int> block_adr;  // nblock + 1 dof boundaries
+  gp<const int> dof_block;  // block of each dof
+  int isl_fstride, isl_istride;  // per-env island scratch (doubles, ints); 0 with one block
   gp<const int> geom_type, geom_body;
   gp<const Real> geom_pos, geom_quat;
   gp<const int> pair_g1, pair_g2;
@@ -145,6 +147,8 @@ struct DState {
   gp<Real> stage;     // n x npair x MAXHIT x CW
   gp<Real> gjac;      // n x nrmax x nv
   gp<Real> grow;      // n x nrmax x 10
+  gp<Real> isl_f;     // n x isl_fstride: gathered island problems
+  gp<int> isl_i;      // n x isl_istride: row block masks, gathered indices, islands
 };
 
 }  // namespace phk

@@ -167,6 +167,12 @@ static void compute_dof_blocks(HostModel& m) {
   if (nv == 0) m.block_adr.clear();
 }
 
+static void compute_dof_block(HostModel& m) {
+  m.dof_block.assign(m.d.nv, 0);
+  for (size_t b = 0; b + 1 < m.block_adr.size(); b++)
+    for (int k = m.block_adr[b]; k < m.block_adr[b + 1]; k++) m.dof_block[k] = (int)b;
+}
+
 bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
   m.d = s;
   int nb = s.nbody, nj = s.njnt, nv = s.nv, ng = s.ngeom, nu = s.nu;
@@ -252,6 +258,7 @@ bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
     m.body_lastdof[b] = last;
   }
   compute_dof_blocks(m);
+  compute_dof_block(m);
   // weld ids (model.rs:301-306)
   m.body_weldid.resize(nb);
   for (int b = 0; b < nb; b++) m.body_weldid[b] = b;
@@ -382,6 +389,9 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
   M.ncmax = std::max(hm.ncmax, 1);
   if (max_contacts > 0) M.ncmax = std::min(M.ncmax, max_contacts);
   M.nrmax = std::max(4 * M.ncmax + 2 * hm.nlimited + d.neq, 1);
+  // island scratch (newton.rs:378-473), only when several dof blocks exist
+  M.isl_fstride = M.nblock > 1 ? d.nv * d.nv + M.nrmax * (d.nv + 2) + 3 * d.nv : 0;
+  M.isl_istride = M.nblock > 1 ? 2 * M.nrmax + d.nv + 2 * M.nblock + 1 : 0;
   // Rows beyond nrl live in HBM. Keep up to 32 in LDS, fewer if that keeps
   // the per-env working set within 10 KiB (four 4-env workgroups per CU).
   M.nrl = std::min(M.nrmax, 32);

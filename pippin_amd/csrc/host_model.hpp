@@ -28,7 +28,7 @@ struct HostModel {
       moment_coef, eq_polycoef, eq_solref, eq_solimp, qpos0, qpos_reset;
 
   // derived (model.rs:279-316)
-  std::vector<int32_t> body_lastdof, dof_parent, body_weldid, block_adr;
+  std::vector<int32_t> body_lastdof, dof_parent, body_weldid, block_adr, dof_block;
   std::vector<uint64_t> body_chain, dof_anc;
   std::vector<int32_t> level_adr, level_body;
   // leaf-to-root accumulation schedule (crba / rne): levels deepest first;

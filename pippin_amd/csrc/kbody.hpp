@@ -782,12 +782,22 @@ PHD int build_rows(const MT& M, const DState& S, Env& E) {
   return nr;
 }
 
-// ---------------- Newton, newton.rs:224-371 (single island) ----------------
-template <class X, class MT>
-PHD void mul_m(const MT& M, Env& E, const Real* x, Real* y) {
-  const int nv = M.nv;
-  const Real* qm = LD(qm);
-  X::par(nv, [&](int i) { y[i] = sdot_n(qm + i * nv, x, nv) + 0.0 * x[i]; });
+// ---------------- Newton, newton.rs:224-371 ----------------
+// One Newton problem: the whole model (one island) or one island gathered
+// into its own arrays (newton.rs:436-471).
+struct NP {
+  int nv, nr, n_ineq;                  // dofs, rows, rows below n_ineq are inequalities
+  const Real *qm, *J, *aref, *dd;      // qm: nv x nv, J: nr x nv (row stride nv)
+  const Real* warm;                    // previous acceleration (warm start candidate)
+  Real* a;                             // in: unconstrained acceleration; out: solution
+  Real* qf;                            // out: constraint force, nv
+  Real* fac;                           // nv x nv factor storage
+};
+
+template <class X>
+PHD void mul_m(const NP& P, const Real* x, Real* y) {
+  const int nv = P.nv;
+  X::par(nv, [&](int i) { y[i] = sdot_n(P.qm + i * nv, x, nv) + 0.0 * x[i]; });
 }
 
 // newton.rs:224-371 for nv <= 16 and nr <= 16 rows held in LDS, with the
@@ -797,18 +807,16 @@ PHD void mul_m(const MT& M, Env& E, const Real* x, Real* y) {
 // per lane directly into the register Cholesky. Every sum keeps the
 // reference's order, so the result equals newton_dense's bit for bit.
 template <class X, class MT>
-PHD int newton_lanes(const MT& M, Env& E, int nr) {
+PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
   using LV = typename X::LV;
-  const int nv = M.nv;
-  const int n_ineq = nr - M.neq;
-  // rows are in LDS on this path (nr <= nrl): address them through the LDS
-  // layout so every access is an LDS access
-  const Real* qm = LD(qm);
-  const Real* J = LD(J);
-  const Real* rdd = LD(dd);
-  Real* Lf = LD(chol);
+  const int nv = P.nv, nr = P.nr, n_ineq = P.n_ineq;
+  // every array of P is in LDS on this path
+  const Real* qm = P.qm;
+  const Real* J = P.J;
+  const Real* rdd = P.dd;
+  Real* Lf = P.fac;
   const LV aref = X::gen([&](int l) {
-    Real v = LD(aref)[clampi(l, nr)];
+    Real v = P.aref[clampi(l, nr)];
     return l < nr ? v : 0.0;
   });
   const LV dd = X::gen([&](int l) {
@@ -885,13 +893,13 @@ PHD int newton_lanes(const MT& M, Env& E, int nr) {
   };
 
   const LV a0 = X::gen([&](int l) {
-    Real v = LD(a)[clampi(l, nv)];
+    Real v = P.a[clampi(l, nv)];
     return l < nv ? v : 0.0;
   });
   LV a = a0;
   {
     const LV warm = X::gen([&](int l) {
-      Real v = LD(qacc)[clampi(l, nv)];
+      Real v = P.warm[clampi(l, nv)];
       return l < nv ? v : 0.0;
     });
     Real c0 = cost(a0, a0), c1 = cost(warm, a0);
@@ -984,8 +992,8 @@ PHD int newton_lanes(const MT& M, Env& E, int nr) {
   const LV f = add_rows(X::splat(0.0), (-dd) * jar, active_mask(jar));
   X::each([&](int l) {
     if (l < nv) {
-      LD(a)[l] = X::at(a, l);
-      LD(qfrc_c)[l] = X::at(f, l);
+      P.a[l] = X::at(a, l);
+      P.qf[l] = X::at(f, l);
     }
   });
   X::sync();
@@ -993,21 +1001,20 @@ PHD int newton_lanes(const MT& M, Env& E, int nr) {
 }
 
 template <class X, class MT>
-PHD int newton_dense(const MT& M, Env& E, int nr) {
-  if (M.nv <= GL && nr <= GL && nr <= M.nrl) return newton_lanes<X>(M, E, nr);
-  const int nv = M.nv;
-  Real* a = LD(a);
+PHD int newton_dense(const MT& M, Env& E, const NP& P) {
+  const int nv = P.nv, nr = P.nr;
+  Real* a = P.a;
   Real* a0 = LD(a0);
   Real* grad = LD(grad);
   Real* p = LD(p);
   Real* tmp = LD(tmp);
   Real* tmp2 = LD(tmp2);
   Real* mp = LD(mp);
-  Real* H = LD(H);
+  Real* H = P.fac;
   Real* sc = LD(scal);
   int* isc = E.si + M.L.i_iscal;
-  const Real* qm = LD(qm);
-  const Real* J = E.J;
+  const Real* qm = P.qm;
+  const Real* J = P.J;
   Real* jar = E.jar;
   Real* jp = E.jp;
   Real* act = E.ract;
@@ -1017,25 +1024,25 @@ PHD int newton_dense(const MT& M, Env& E, int nr) {
   // cost(x) = 1/2 (x-a0)'M(x-a0) + sum_active 1/2 D r^2 with r = J x - aref
   auto cost_into = [&](const Real* x, int slot) {
     X::par(nv, [&](int k) { tmp[k] = x[k] - a0[k]; });
-    mul_m<X>(M, E, tmp, tmp2);
-    X::par(nr, [&](int r) { jar[r] = sdot_n(J + r * nv, x, nv) - E.aref[r]; });
+    mul_m<X>(P, tmp, tmp2);
+    X::par(nr, [&](int r) { jar[r] = sdot_n(J + r * nv, x, nv) - P.aref[r]; });
     X::par(1, [&](int) {
       Real s = rsum0();
       for (int k = 0; k < nv; k++) s += tmp[k] * tmp2[k];
       Real c = 0.5 * s;
       for (int r = 0; r < nr; r++) {
-        bool on = r >= nr - M.neq || jar[r] < 0.0;
-        if (on) c += 0.5 * E.dd[r] * jar[r] * jar[r];
+        bool on = r >= P.n_ineq || jar[r] < 0.0;
+        if (on) c += 0.5 * P.dd[r] * jar[r] * jar[r];
       }
       sc[slot] = c;
     });
   };
   // warm start from the previous acceleration when cheaper (newton.rs:238-245)
-  const Real* warm = LD(qacc);
+  const Real* warm = P.warm;
   cost_into(a0, 0);
   cost_into(warm, 1);
   if (sc[1] < sc[0]) X::par(nv, [&](int k) { a[k] = warm[k]; });
-  mul_m<X>(M, E, a0, tmp);
+  mul_m<X>(P, a0, tmp);
   X::par(1, [&](int) {
     Real s = rsum0();
     for (int k = 0; k < nv; k++) s += tmp[k] * tmp[k];
@@ -1044,19 +1051,19 @@ PHD int newton_dense(const MT& M, Env& E, int nr) {
   const Real scale = sc[2];
   const Real tol = M.tolerance;
   const int cap = M.iterations > 1 ? M.iterations : 1;
-  const int n_ineq = nr - M.neq;
+  const int n_ineq = P.n_ineq;
   X::par(nr, [&](int r) { prev[r] = -1.0; });  // u128::MAX sentinel
   int iters = 0;
   while (iters < cap) {
     iters++;
     X::par(nv, [&](int k) { tmp[k] = a[k] - a0[k]; });
-    mul_m<X>(M, E, tmp, grad);
+    mul_m<X>(P, tmp, grad);
     X::par(nr, [&](int r) {
-      Real v = sdot_n(J + r * nv, a, nv) - E.aref[r];
+      Real v = sdot_n(J + r * nv, a, nv) - P.aref[r];
       jar[r] = v;
       bool on = r >= n_ineq || v < 0.0;
       act[r] = on ? 1.0 : 0.0;
-      E.rf[r] = E.dd[r] * v;
+      E.rf[r] = P.dd[r] * v;
     });
     X::par(nv, [&](int k) {
       Real g = grad[k];
@@ -1088,7 +1095,7 @@ PHD int newton_dense(const MT& M, Env& E, int nr) {
         if (act[r] == 0.0) continue;
         Real ji = J[r * nv + i];
         if (ji == 0.0) continue;
-        h += (E.dd[r] * ji) * J[r * nv + j];
+        h += (P.dd[r] * ji) * J[r * nv + j];
       }
       H[t] = h;
     });
@@ -1097,7 +1104,7 @@ PHD int newton_dense(const MT& M, Env& E, int nr) {
     X::par(nv, [&](int k) { p[k] = -grad[k]; });
     chol_solve_block<X>(M, E, H, nv, 0, nv, p);
     // exact line search along p (newton.rs:310-351)
-    mul_m<X>(M, E, p, mp);
+    mul_m<X>(P, p, mp);
     X::par(nr, [&](int r) { jp[r] = sdot_n(J + r * nv, p, nv); });
     X::par(1, [&](int) {
       Real mpp = rsum0(), mpg = rsum0();
@@ -1108,7 +1115,7 @@ PHD int newton_dense(const MT& M, Env& E, int nr) {
         for (int r = 0; r < nr; r++) {
           Real v = jar[r] + al * jp[r];
           if (r >= n_ineq || v < 0.0) {
-            Real dq = E.dd[r] * jp[r];
+            Real dq = P.dd[r] * jp[r];
             d1 += dq * v;
             d2 += dq * jp[r];
           }
@@ -1140,16 +1147,128 @@ PHD int newton_dense(const MT& M, Env& E, int nr) {
   (void)nb;
   // add_constraint_force (newton.rs:361-371)
   X::par(nr, [&](int r) {
-    Real v = sdot_n(J + r * nv, a, nv) - E.aref[r];
+    Real v = sdot_n(J + r * nv, a, nv) - P.aref[r];
     act[r] = (r >= n_ineq || v < 0.0) ? 1.0 : 0.0;
-    E.rf[r] = -E.dd[r] * v;
+    E.rf[r] = -P.dd[r] * v;
   });
   X::par(nv, [&](int k) {
     Real f = 0.0;
     for (int r = 0; r < nr; r++)
       if (act[r] != 0.0) f += E.rf[r] * J[r * nv + k];
-    LD(qfrc_c)[k] = f;
+    P.qf[k] = f;
   });
+  return iters;
+}
+
+// newton.rs:378-473. Rows join the dof blocks they touch into islands
+// (union-find; an island is named by its smallest block, which is where the
+// reference's parent[max] = min links leave the root). One island spanning
+// every block is solved in place; otherwise each island, in increasing order,
+// is gathered (its dofs block by block, its rows in order), solved, and
+// scattered back. Iterations: the maximum over islands.
+template <class X, class MT>
+PHD int newton_solve(const MT& M, const DState& S, Env& E, int nr) {
+  const int nv = M.nv;
+  const int n_ineq = nr - M.neq;
+  auto whole = [&]() {
+    if (nv <= GL && nr <= GL && nr <= M.nrl) {
+      NP P{nv, nr, n_ineq, LD(qm), LD(J), LD(aref), LD(dd), LD(qacc), LD(a), LD(qfrc_c), LD(chol)};
+      return newton_lanes<X>(M, E, P);
+    }
+    NP P{nv, nr, n_ineq, LD(qm), E.J, E.aref, E.dd, LD(qacc), LD(a), LD(qfrc_c), LD(chol)};
+    return newton_dense<X>(M, E, P);
+  };
+  if (M.nblock <= 1) return whole();
+  // per-env island scratch in HBM
+  Real* fs = (S.isl_f + E.e * (int64_t)M.isl_fstride).p;
+  int* is = (S.isl_i + E.e * (int64_t)M.isl_istride).p;
+  int* rmask = is;                 // nrmax: blocks each row touches
+  int* sidx = rmask + M.nrmax;     // nv: gathered dofs
+  int* sridx = sidx + nv;          // nrmax: gathered rows
+  int* root = sridx + M.nrmax;     // nblock: island of each block
+  int* isl = root + M.nblock;      // nblock + 1: island list, then its length
+  Real* sqm = fs;
+  Real* sJ = sqm + nv * nv;
+  Real* saref = sJ + (int64_t)M.nrmax * nv;
+  Real* sdd = saref + M.nrmax;
+  Real* sa = sdd + M.nrmax;
+  Real* swarm = sa + nv;
+  Real* sqf = swarm + nv;
+  int* isc = E.si + M.L.i_iscal;
+  const Real* J = E.J;
+  X::par(nr, [&](int r) {
+    uint32_t m = 0;
+    for (int k = 0; k < nv; k++)
+      if (J[r * nv + k] != 0.0) m |= 1u << M.dof_block[k];
+    rmask[r] = (int)m;
+  });
+  X::par(1, [&](int) {
+    const int nb = M.nblock;
+    for (int b = 0; b < nb; b++) root[b] = b;
+    auto find = [&](int x) {
+      while (root[x] != x) x = root[x];
+      return x;
+    };
+    for (int r = 0; r < nr; r++) {
+      uint32_t m = (uint32_t)rmask[r];
+      if (!m) continue;
+      int first = __builtin_ctz(m);
+      for (int b = first + 1; b < nb; b++)
+        if ((m >> b) & 1u) {
+          int x = find(first), y = find(b);
+          if (x != y) root[x > y ? x : y] = x < y ? x : y;
+        }
+    }
+    for (int b = 0; b < nb; b++) root[b] = find(b);
+    // islands holding a row, in increasing order
+    int ni = 0;
+    for (int b = 0; b < nb; b++) {
+      bool used = false;
+      for (int r = 0; r < nr && !used; r++) used = rmask[r] && root[__builtin_ctz((uint32_t)rmask[r])] == b;
+      if (used) isl[ni++] = b;
+    }
+    bool all = ni == 1;
+    for (int b = 0; b < nb && all; b++) all = root[b] == isl[0];
+    isl[nb] = all ? -1 : ni;
+  });
+  if (isl[M.nblock] < 0) return whole();
+  const int ni = isl[M.nblock];
+  int iters = 0;
+  for (int ii = 0; ii < ni; ii++) {
+    X::par(1, [&](int) {
+      const int id = isl[ii];
+      int k = 0, q = 0, qi = 0;
+      for (int b = 0; b < M.nblock; b++)
+        if (root[b] == id)
+          for (int d = M.block_adr[b]; d < M.block_adr[b + 1]; d++) sidx[k++] = d;
+      for (int r = 0; r < nr; r++)
+        if (rmask[r] && root[__builtin_ctz((uint32_t)rmask[r])] == id) {
+          sridx[q++] = r;
+          if (r < n_ineq) qi++;
+        }
+      isc[10] = k;
+      isc[11] = q;
+      isc[12] = qi;
+    });
+    const int k = isc[10], q = isc[11], qi = isc[12];
+    X::par(k * k, [&](int t) { sqm[t] = LD(qm)[sidx[t / k] * nv + sidx[t % k]]; });
+    X::par(q * k, [&](int t) { sJ[t] = J[sridx[t / k] * nv + sidx[t % k]]; });
+    X::par(q, [&](int t) {
+      saref[t] = E.aref[sridx[t]];
+      sdd[t] = E.dd[sridx[t]];
+    });
+    X::par(k, [&](int t) {
+      sa[t] = LD(a)[sidx[t]];
+      swarm[t] = LD(qacc)[sidx[t]];
+    });
+    NP P{k, q, qi, sqm, sJ, saref, sdd, swarm, sa, sqf, LD(chol)};
+    int it = (k <= GL && q <= GL) ? newton_lanes<X>(M, E, P) : newton_dense<X>(M, E, P);
+    iters = it > iters ? it : iters;
+    X::par(k, [&](int t) {
+      LD(a)[sidx[t]] = sa[t];
+      LD(qfrc_c)[sidx[t]] = sqf[t];
+    });
+  }
   return iters;
 }
 
@@ -1210,7 +1329,7 @@ PHD bool step_env(const MT& M, const DState& S, Env& E) {
   }
   X::par(1, [&](int) { isc[9] = nr; });
   if (nr > 0) {
-    int it = newton_dense<X>(M, E, nr);
+    int it = newton_solve<X>(M, S, E, nr);
     X::par(1, [&](int) { isc[8] = it; });
   }
   const bool damped = M.v_damped[E.var] != 0;

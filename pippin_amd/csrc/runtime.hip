@@ -224,8 +224,8 @@ struct ph_sim {
   DBuf<int> dv_damped;
   // state buffers
   DBuf<double> s_qpos, s_qvel, s_ctrl, s_qacc, s_applied, s_time, s_gxpos, s_gxmat, s_xpos, s_xmat, s_con, s_pcon,
-      s_plambda, s_stage, s_gjac, s_grow;
-  DBuf<int> s_ncon, s_iters, s_nrows, s_status, s_cgeom, s_pcgeom, s_npcon;
+      s_plambda, s_stage, s_gjac, s_grow, s_islf;
+  DBuf<int> s_ncon, s_iters, s_nrows, s_status, s_cgeom, s_pcgeom, s_npcon, s_isli;
   DBuf<int64_t> d_ids;
   DBuf<DModel> d_model;  // device copies of M and S, read through the scalar cache
   DBuf<DState> d_state;
@@ -349,10 +349,8 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   int solver = d.solver_kind;
   if (opt && opt->solver_override >= 0) solver = opt->solver_override;
   if (solver != PH_SOLVER_NEWTON) return fail(PH_ERR_MODEL, "the device path implements the Newton solver only");
-  if (d.neq > 0 || hm.block_adr.size() > 2) {
-    // islands over several dof blocks (newton.rs:391-470) are not on the device yet
-    if (hm.block_adr.size() > 2) return fail(PH_ERR_MODEL, "models with several dof blocks are not supported yet");
-  }
+  if (hm.block_adr.size() > 33)  // island masks are 32-bit
+    return fail(PH_ERR_MODEL, "models with more than 32 dof blocks are not supported on the device");
   s->n = num_envs;
   s->device = opt ? opt->device : 0;
   HIPCHK(hipSetDevice(s->device));
@@ -413,6 +411,7 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   HIPCHK(ui(hm.dof_parent, &M.dof_parent));
   HIPCHK(ui(hm.dof_jnt, &M.dof_jnt));
   HIPCHK(ui(hm.block_adr, &M.block_adr));
+  HIPCHK(ui(hm.dof_block, &M.dof_block));
   HIPCHK(ui(hm.geom_type, &M.geom_type));
   HIPCHK(ui(hm.geom_body, &M.geom_body));
   HIPCHK(ui(hm.pair_g1, &M.pair_g1));
@@ -485,6 +484,8 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   HIPCHK(s->s_stage.alloc(n * std::max(M.npair, 1) * MAXHIT * CW));
   HIPCHK(s->s_gjac.alloc(n * (int64_t)M.nrmax * d.nv));
   HIPCHK(s->s_grow.alloc(n * (int64_t)M.nrmax * 10));
+  HIPCHK(s->s_islf.alloc(std::max<int64_t>(n * (int64_t)M.isl_fstride, 1)));
+  HIPCHK(s->s_isli.alloc(std::max<int64_t>(n * (int64_t)M.isl_istride, 1)));
   for (auto* b : {&s->s_ncon, &s->s_iters, &s->s_nrows, &s->s_status, &s->s_npcon}) {
     HIPCHK(b->alloc(n));
     HIPCHK(b->zero());
@@ -512,6 +513,8 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   S.stage = s->s_stage.p;
   S.gjac = s->s_gjac.p;
   S.grow = s->s_grow.p;
+  S.isl_f = s->s_islf.p;
+  S.isl_i = s->s_isli.p;
 
   // invweight0 (forward::set_const) for the base variant, then poses (PippinCpu::new runs kinematics)
   if (int rc = run_setconst(s.get(), {0})) return rc;

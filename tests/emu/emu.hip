@@ -35,7 +35,8 @@ struct emu_sim {
   std::vector<uint32_t> v_ct, v_ca;
   std::vector<int> v_damped;
   std::vector<double> qpos, qvel, ctrl, qacc, applied, time, gxpos, gxmat, xpos, xmat, con, pcon, plambda, stage, gjac,
-      grow;
+      grow, islf;
+  std::vector<int> isli;
   std::vector<int> ncon, iters, nrows, status, cgeom, pcgeom, npcon;
   std::vector<double> lds;
 };
@@ -88,6 +89,7 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   M.dof_anc = hm.dof_anc.data();
   M.dof_armature = hm.dof_armature.data();
   M.block_adr = hm.block_adr.data();
+  M.dof_block = hm.dof_block.data();
   M.geom_type = hm.geom_type.data();
   M.geom_body = hm.geom_body.data();
   M.geom_pos = hm.geom_pos.data();
@@ -168,6 +170,8 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   s->stage.assign(nn * (M.npair > 0 ? M.npair : 1) * MAXHIT * CW, 0.0);
   s->gjac.assign(nn * (int64_t)M.nrmax * d.nv, 0.0);
   s->grow.assign(nn * (int64_t)M.nrmax * 10, 0.0);
+  s->islf.assign(std::max<int64_t>(nn * (int64_t)M.isl_fstride, 1), 0.0);
+  s->isli.assign(std::max<int64_t>(nn * (int64_t)M.isl_istride, 1), 0);
   s->ncon.assign(nn, 0);
   s->iters.assign(nn, 0);
   s->nrows.assign(nn, 0);
@@ -198,6 +202,8 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   S.stage = s->stage.data();
   S.gjac = s->gjac.data();
   S.grow = s->grow.data();
+  S.isl_f = s->islf.data();
+  S.isl_i = s->isli.data();
   s->lds.assign(M.L.bytes / sizeof(double) + 16, 0.0);
   // set_const for the base variant
   Env E;

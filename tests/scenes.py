@@ -17,3 +17,20 @@ PRIMITIVE_SCENES = {
 
 def primitive_xml(name: str) -> str:
     return f'<mujoco><option timestep="0.004"/><worldbody>{PRIMITIVE_SCENES[name]}</worldbody></mujoco>'
+
+
+# several free bodies: one dof block each, islands formed by contacts
+MULTIBODY_SCENES = {
+    "box_drop": None,  # pippin_amd/assets/box_drop.xml
+    "stack3": '<mujoco><option timestep="0.004"/><worldbody><geom type="plane" size="5 5 .1"/>'
+              '<body pos="0 0 .06"><freejoint/><geom type="box" size=".05 .05 .05"/></body>'
+              '<body pos=".01 0 .17" euler="0 0 20"><freejoint/><geom type="box" size=".05 .05 .05"/></body>'
+              '<body pos="0.3 0 .1"><freejoint/><geom type="sphere" size=".05"/></body>'
+              '</worldbody></mujoco>',
+    "pile": '<mujoco><option timestep="0.004"/><worldbody><geom type="plane" size="5 5 .1"/>'
+            '<body pos="0 0 .05"><freejoint/><geom type="capsule" size=".03 .1" euler="0 90 0"/></body>'
+            '<body pos="0 0 .12" euler="0 90 30"><freejoint/><geom type="capsule" size=".03 .1"/></body>'
+            '<body pos=".05 .05 .2"><freejoint/><geom type="sphere" size=".04"/></body>'
+            '<body pos="-.4 0 .2"><freejoint/><geom type="box" size=".04 .03 .05"/></body>'
+            '</worldbody></mujoco>',
+}

@@ -13,7 +13,7 @@ import pytest
 from oracle import mjcf
 from oracle.pyoracle import OracleSim
 from tests.emu.pyemu import EmuSim
-from tests.scenes import PRIMITIVE_SCENES, primitive_xml
+from tests.scenes import MULTIBODY_SCENES, PRIMITIVE_SCENES, primitive_xml
 
 
 def _perturbed(q, rng, scale, free=True):
@@ -115,3 +115,26 @@ def test_primitive_pairs_exact(name):
         co, ce = o.contacts(k), e.contacts(k)
         for f in ("geom1", "geom2", "feature", "depth", "pos", "normal", "tangent"):
             assert np.array_equal(co[f], ce[f]), f
+
+
+@pytest.mark.parametrize("name", sorted(MULTIBODY_SCENES))
+def test_multibody_islands_exact(assets, name):
+    """Several dof blocks: Newton islands formed by contacts (newton.rs:378-473)."""
+    xml = MULTIBODY_SCENES[name]
+    arr = mjcf.load_file(assets / "box_drop.xml") if xml is None else mjcf.load_string(xml)
+    n = 6
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(3)
+    nb = (int(arr["nq"]) // 7)
+    q = o.get("qpos")
+    for b in range(nb):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.01
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+    for t in range(300):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert np.array_equal(o.solver_iterations(), e.solver_iterations())
+    assert o.ncon().sum() > 0

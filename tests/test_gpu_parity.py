@@ -15,7 +15,7 @@ import pytest
 from oracle import mjcf
 from oracle.pyoracle import OracleSim, cr_sincos
 from pippin_amd.physics import HipPhysics, selftest_math
-from tests.scenes import PRIMITIVE_SCENES, primitive_xml
+from tests.scenes import MULTIBODY_SCENES, PRIMITIVE_SCENES, primitive_xml
 
 pytestmark = pytest.mark.gpu
 
@@ -151,3 +151,28 @@ def test_specialised_kernel_equals_generic(assets, monkeypatch):
     assert np.array_equal(sims[0].get("qpos"), sims[1].get("qpos"))
     assert np.array_equal(sims[0].get("qvel"), sims[1].get("qvel"))
     assert np.array_equal(sims[0].solver_iterations(), sims[1].solver_iterations())
+
+
+@pytest.mark.parametrize("name", sorted(MULTIBODY_SCENES))
+def test_multibody_islands_match(assets, name):
+    """Several dof blocks: Newton islands formed by contacts (newton.rs:378-473)."""
+    xml = MULTIBODY_SCENES[name]
+    arr = mjcf.load_file(assets / "box_drop.xml") if xml is None else mjcf.load_string(xml)
+    n = 64
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(3)
+    nb = int(arr["nq"]) // 7
+    q = o.get("qpos")
+    for b in range(nb):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.01
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    say("multibody islands", name)
+    for t in range(300):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        err = np.abs(o.get("qpos") - g.get("qpos")).max()
+        assert err <= 1e-9, (t, err)
+    assert np.array_equal(o.get("qpos"), g.get("qpos"))
+    assert np.array_equal(o.solver_iterations(), g.solver_iterations())
