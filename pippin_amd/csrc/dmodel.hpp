@@ -147,6 +147,7 @@ struct DState {
   gp<Real> stage;     // n x npair x MAXHIT x CW
   gp<Real> gjac;      // n x nrmax x nv
   gp<Real> grow;      // n x nrmax x 10
+  gp<Real> gmj;       // n x nrmax x nv: PGS M^-1 J^T rows
   gp<Real> isl_f;     // n x isl_fstride: gathered island problems
   gp<int> isl_i;      // n x isl_istride: row block masks, gathered indices, islands
 };

@@ -1272,6 +1272,266 @@ PHD int newton_solve(const MT& M, const DState& S, Env& E, int nr) {
   return iters;
 }
 
+// ---------------- PGS, solver.rs:44-289 ----------------
+// math.rs:449-463 on one lane: the substitutions in the reference's order.
+PHD void solve_serial(const Real* L, int nv, int s0, int e, Real* x) {
+  for (int i = s0; i < e; i++) x[i] = (x[i] - sdot_n(L + i * nv + s0, x + s0, i - s0)) / L[i * nv + i];
+  for (int i = e - 1; i >= s0; i--) {
+    x[i] /= L[i * nv + i];
+    Real xi = x[i];
+    const Real* li = L + i * nv + s0;
+    for (int k = 0; k < i - s0; k++) x[s0 + k] += -xi * li[k];
+  }
+}
+
+// Row record of the PGS problem (HBM, 10 doubles per row).
+enum { PR_SEP = 0, PR_MU, PR_INVA, PR_LAM, PR_KIND, PR_NORMAL, PR_CONTACT };
+enum { PK_NORMAL = 0, PK_FRICTION = 1, PK_BILATERAL = 2 };
+
+// solver.rs:63-251. v: velocity after the smooth update (in, out); vpos:
+// the biased velocity the positions integrate with. Rows, M^-1 J^T and the
+// row records live in HBM; a Gauss-Seidel sweep is sequential over rows, so
+// one lane per env runs the sweeps while the rows, the M^-1 J^T columns and
+// the constraint force are formed lane-parallel.
+template <class X, class MT>
+PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
+  const int nv = M.nv;
+  const Real h = M.timestep;
+  int* isc = E.si + M.L.i_iscal;
+  auto con = S.con + E.e * (int64_t)M.ncmax * CW;
+  auto cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
+  Real* J = (S.gjac + E.e * (int64_t)M.nrmax * nv).p;
+  Real* mj = (S.gmj + E.e * (int64_t)M.nrmax * nv).p;
+  Real* rw = (S.grow + E.e * (int64_t)M.nrmax * 10).p;
+  const Real* qpos = LD(qpos);
+  const int ncon = isc[0];
+  // contact rows: normal, then the two friction directions (solver.rs:67-91)
+  X::par(ncon * nv, [&](int t) {
+    int c = t / nv, dof = t % nv;
+    auto cc = con + c * CW;
+    int b1 = M.geom_body[cg[3 * c]], b2 = M.geom_body[cg[3 * c + 1]];
+    V3 t1, t2;
+    contact_frame(cc, &t1, &t2);
+    V3 p = ld3(cc);
+    V3 dirs[3] = {ld3(cc + 3), t1, t2};
+    SP cd = ldsp(LD(cdof) + 6 * dof);
+    bool on2 = (M.body_chain[b2] >> dof) & 1, on1 = (M.body_chain[b1] >> dof) & 1;
+    for (int r = 0; r < 3; r++) {
+      Real jv = 0.0;
+      if (on2) jv += 1.0 * dot(dirs[r], point_velocity(cd, p));
+      if (on1) jv += -1.0 * dot(dirs[r], point_velocity(cd, p));
+      J[(3 * c + r) * nv + dof] = jv;
+    }
+  });
+  X::par(ncon, [&](int c) {
+    int g1 = cg[3 * c], g2 = cg[3 * c + 1];
+    Real mu = rmax(M.v_friction[(int64_t)E.var * M.ngeom + g1], M.v_friction[(int64_t)E.var * M.ngeom + g2]);
+    for (int r = 0; r < 3; r++) {
+      Real* q = rw + (3 * c + r) * 10;
+      q[PR_SEP] = -con[c * CW + 9];
+      q[PR_MU] = mu;
+      q[PR_INVA] = 0.0;
+      q[PR_LAM] = 0.0;
+      q[PR_KIND] = r == 0 ? PK_NORMAL : PK_FRICTION;
+      q[PR_NORMAL] = 3 * c;
+      q[PR_CONTACT] = c;
+    }
+  });
+  // limits within reach of this step, then equalities (solver.rs:92-127)
+  X::par(1, [&](int) {
+    int r = 3 * ncon;
+    auto row = [&](Real sep, int kind) {
+      Real* q = rw + r * 10;
+      q[PR_SEP] = sep;
+      q[PR_MU] = 0.0;
+      q[PR_INVA] = 0.0;
+      q[PR_LAM] = 0.0;
+      q[PR_KIND] = kind;
+      q[PR_NORMAL] = 0;
+      q[PR_CONTACT] = -1;
+      r++;
+    };
+    for (int j = 0; j < M.njnt; j++) {
+      int t = M.jnt_type[j];
+      if (!M.jnt_limited[j] || !(t == J_HINGE || t == J_SLIDE)) continue;
+      Real q = qpos[M.jnt_qposadr[j]];
+      int da = M.jnt_dofadr[j];
+      Real reach = M.margin + 2.0 * h * fabs(v[da]);
+      Real seps[2] = {q - M.jnt_range[2 * j], M.jnt_range[2 * j + 1] - q}, signs[2] = {1.0, -1.0};
+      for (int sg = 0; sg < 2; sg++)
+        if (seps[sg] < reach) {
+          if (r >= M.nrmax) {
+            isc[7] |= ST_ROW_OVERFLOW;
+            return;
+          }
+          for (int k = 0; k < nv; k++) J[r * nv + k] = 0.0;
+          J[r * nv + da] = signs[sg];
+          row(seps[sg], PK_NORMAL);
+        }
+    }
+    for (int e = 0; e < M.neq; e++) {
+      if (r >= M.nrmax) {
+        isc[7] |= ST_ROW_OVERFLOW;
+        return;
+      }
+      int j1 = M.eq_j1[e], j2 = M.eq_j2[e];
+      auto c = M.eq_poly + 5 * e;
+      Real q1 = qpos[M.jnt_qposadr[j1]] - M.qpos0[M.jnt_qposadr[j1]];
+      for (int k = 0; k < nv; k++) J[r * nv + k] = 0.0;
+      J[r * nv + M.jnt_dofadr[j1]] += 1.0;
+      Real poly = c[0];
+      if (j2 >= 0) {
+        Real x = qpos[M.jnt_qposadr[j2]] - M.qpos0[M.jnt_qposadr[j2]];
+        poly = c[0] + x * (c[1] + x * (c[2] + x * (c[3] + x * c[4])));
+        J[r * nv + M.jnt_dofadr[j2]] -= c[1] + x * (2.0 * c[2] + x * (3.0 * c[3] + x * 4.0 * c[4]));
+      }
+      row(q1 - poly, PK_BILATERAL);
+    }
+    isc[9] = r;
+  });
+  const int nr = isc[9];
+  if (isc[7] & ST_ROW_OVERFLOW) return;
+  if (nr == 0) {
+    X::par(nv, [&](int k) {
+      vpos[k] = v[k];
+      LD(qfrc_c)[k] = 0.0;
+    });
+    X::par(1, [&](int) { S.npcon[E.e] = 0; });
+    return;
+  }
+  // M^-1 J_r^T and the row's inverse effective mass, one row per lane
+  X::par(nr, [&](int r) {
+    Real* x = mj + r * nv;
+    for (int k = 0; k < nv; k++) x[k] = J[r * nv + k];
+    for (int blk = 0; blk < M.nblock; blk++) solve_serial(LD(chol), nv, M.block_adr[blk], M.block_adr[blk + 1], x);
+    Real a = rsum0();
+    for (int k = 0; k < nv; k++) a += J[r * nv + k] * x[k];
+    rw[r * 10 + PR_INVA] = a > 1e-12 ? 1.0 / a : 0.0;
+  });
+  // warm start from the matching previous contact (solver.rs:253-289)
+  auto pcon = S.pcon + E.e * (int64_t)M.ncmax * CW;
+  auto pcg = S.pcgeom + E.e * (int64_t)M.ncmax * 3;
+  auto plam = S.plambda + E.e * (int64_t)M.ncmax * 3;
+  const int npc = S.npcon[E.e];
+  X::par(ncon, [&](int c) {
+    if (npc == 0) return;
+    auto cc = con + c * CW;
+    Real bd = 0.01 * 0.01;
+    int bi = -1;
+    V3 cp = ld3(cc);
+    for (int pi = 0; pi < npc; pi++) {
+      if (pcg[3 * pi] == cg[3 * c] && pcg[3 * pi + 1] == cg[3 * c + 1]) {
+        Real d2 = norm2(ld3(pcon + pi * CW) - cp);
+        if (d2 < bd) {
+          bd = d2;
+          bi = pi;
+        }
+      }
+    }
+    if (bi >= 0) {
+      V3 pt1, pt2, t1, t2;
+      contact_frame(pcon + bi * CW, &pt1, &pt2);
+      contact_frame(cc, &t1, &t2);
+      V3 f = pt1 * plam[3 * bi + 1] + pt2 * plam[3 * bi + 2];
+      rw[(3 * c) * 10 + PR_LAM] = plam[3 * bi];
+      rw[(3 * c + 1) * 10 + PR_LAM] = dot(f, t1);
+      rw[(3 * c + 2) * 10 + PR_LAM] = dot(f, t2);
+    }
+  });
+  X::par(nv, [&](int k) {
+    for (int r = 0; r < nr; r++) {
+      Real l = rw[r * 10 + PR_LAM];
+      if (l != 0.0) v[k] += mj[r * nv + k] * l;
+    }
+  });
+  // soft-constraint coefficients (solver.rs:44-50)
+  const Real omega = 2.0 * 3.141592653589793 * M.hertz;
+  const Real a1 = 2.0 * M.zeta + h * omega;
+  const Real a2 = h * omega * a1;
+  const Real a3 = 1.0 / (1.0 + a2);
+  const Real bias_rate = omega / a1, mass_scale = a2 * a3, impulse_scale = a3;
+  const Real max_bias = 1.0, pgs_tol = 1e-5;
+  X::par(1, [&](int) {
+    auto iterate = [&](bool use_bias) {
+      Real max_dv = 0.0;
+      for (int r = 0; r < nr; r++) {
+        Real* q = rw + r * 10;
+        const Real inv_a = q[PR_INVA];
+        if (inv_a == 0.0) continue;
+        const Real* jr = J + r * nv;
+        Real jv = rsum0();
+        for (int k = 0; k < nv; k++) jv += jr[k] * v[k];
+        const Real lam = q[PR_LAM];
+        const int kind = (int)q[PR_KIND];
+        Real nw;
+        if (kind == PK_NORMAL) {
+          Real sp = q[PR_SEP] + M.slop;
+          Real bias, ms, is;
+          if (sp > 0.0) {
+            bias = sp / h;
+            ms = 1.0;
+            is = 0.0;
+          } else if (use_bias) {
+            bias = rmax(bias_rate * sp, -max_bias);
+            ms = mass_scale;
+            is = impulse_scale;
+          } else {
+            bias = 0.0;
+            ms = 1.0;
+            is = 0.0;
+          }
+          Real delta = -inv_a * ms * (jv + bias) - is * lam;
+          nw = rmax(lam + delta, 0.0);
+        } else if (kind == PK_BILATERAL) {
+          Real bias, ms, is;
+          if (use_bias) {
+            bias = rclamp(bias_rate * q[PR_SEP], -max_bias, max_bias);
+            ms = mass_scale;
+            is = impulse_scale;
+          } else {
+            bias = 0.0;
+            ms = 1.0;
+            is = 0.0;
+          }
+          nw = lam - inv_a * ms * (jv + bias) - is * lam;
+        } else {
+          Real bound = q[PR_MU] * rw[(int)q[PR_NORMAL] * 10 + PR_LAM];
+          nw = rclamp(lam - inv_a * jv, -bound, bound);
+        }
+        Real dl = nw - lam;
+        if (dl != 0.0) {
+          max_dv = rmax(max_dv, fabs(dl) / inv_a);
+          const Real* mr = mj + r * nv;
+          for (int k = 0; k < nv; k++) v[k] += mr[k] * dl;
+          q[PR_LAM] = nw;
+        }
+      }
+      return max_dv;
+    };
+    for (int it = 0; it < M.iterations; it++)
+      if (iterate(true) < pgs_tol) break;
+    for (int k = 0; k < nv; k++) vpos[k] = v[k];
+    int relax = M.iterations / 3 > 2 ? M.iterations / 3 : 2;
+    for (int it = 0; it < relax; it++)
+      if (iterate(false) < pgs_tol) break;
+  });
+  // qfrc_constraint = sum_r J_r lambda_r / h, rows in order
+  X::par(nv, [&](int k) {
+    Real f = 0.0;
+    for (int r = 0; r < nr; r++) f += J[r * nv + k] * rw[r * 10 + PR_LAM] / h;
+    LD(qfrc_c)[k] = f;
+  });
+  // history for the next warm start: this step's contacts and their forces
+  X::par(ncon * CW, [&](int t) { pcon[t] = con[t]; });
+  X::par(ncon, [&](int c) {
+    for (int i = 0; i < 3; i++) {
+      pcg[3 * c + i] = cg[3 * c + i];
+      plam[3 * c + i] = rw[(3 * c + i) * 10 + PR_LAM];
+    }
+  });
+  X::par(1, [&](int) { S.npcon[E.e] = ncon; });
+}
+
 // ---------------- integrate_pos, forward.rs:459-480 ----------------
 template <class X, class MT>
 PHD void integrate_pos(const MT& M, Env& E, const Real* v, Real h) {
@@ -1317,6 +1577,23 @@ PHD bool step_env(const MT& M, const DState& S, Env& E) {
     isc[1] = 0;
     isc[5] = 0;
   });
+  if (M.solver_kind == 1) {
+    // PGS branch of forward.rs:336-389: M + h D, the smooth velocity, then
+    // PGS on velocities; positions integrate with the biased velocity
+    if (!factor_solve_mass<X>(M, E, h, a)) {
+      X::par(1, [&](int) { isc[7] |= ST_NOTPD; });
+      return false;
+    }
+    Real* vel = LD(vel);
+    Real* vpos = LD(acc);
+    X::par(nv, [&](int k) { vel[k] = LD(qvel)[k] + h * a[k]; });
+    pgs_solve<X>(M, S, E, vel, vpos);
+    if (isc[7] & ST_ROW_OVERFLOW) return false;
+    X::par(nv, [&](int k) { LD(qacc)[k] = (vel[k] - LD(qvel)[k]) / h; });
+    integrate_pos<X>(M, E, vpos, h);
+    X::par(nv, [&](int k) { LD(qvel)[k] = vel[k]; });
+    return true;
+  }
   if (!factor_solve_mass<X>(M, E, 0.0, a)) {
     X::par(1, [&](int) { isc[7] |= ST_NOTPD; });
     return false;

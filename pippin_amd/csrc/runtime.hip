@@ -224,7 +224,7 @@ struct ph_sim {
   DBuf<int> dv_damped;
   // state buffers
   DBuf<double> s_qpos, s_qvel, s_ctrl, s_qacc, s_applied, s_time, s_gxpos, s_gxmat, s_xpos, s_xmat, s_con, s_pcon,
-      s_plambda, s_stage, s_gjac, s_grow, s_islf;
+      s_plambda, s_stage, s_gjac, s_grow, s_islf, s_gmj;
   DBuf<int> s_ncon, s_iters, s_nrows, s_status, s_cgeom, s_pcgeom, s_npcon, s_isli;
   DBuf<int64_t> d_ids;
   DBuf<DModel> d_model;  // device copies of M and S, read through the scalar cache
@@ -348,7 +348,6 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   const ph_model_desc& d = hm.d;
   int solver = d.solver_kind;
   if (opt && opt->solver_override >= 0) solver = opt->solver_override;
-  if (solver != PH_SOLVER_NEWTON) return fail(PH_ERR_MODEL, "the device path implements the Newton solver only");
   if (hm.block_adr.size() > 33)  // island masks are 32-bit
     return fail(PH_ERR_MODEL, "models with more than 32 dof blocks are not supported on the device");
   s->n = num_envs;
@@ -485,6 +484,7 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   HIPCHK(s->s_gjac.alloc(n * (int64_t)M.nrmax * d.nv));
   HIPCHK(s->s_grow.alloc(n * (int64_t)M.nrmax * 10));
   HIPCHK(s->s_islf.alloc(std::max<int64_t>(n * (int64_t)M.isl_fstride, 1)));
+  HIPCHK(s->s_gmj.alloc(M.solver_kind == 1 ? n * (int64_t)M.nrmax * d.nv : 1));
   HIPCHK(s->s_isli.alloc(std::max<int64_t>(n * (int64_t)M.isl_istride, 1)));
   for (auto* b : {&s->s_ncon, &s->s_iters, &s->s_nrows, &s->s_status, &s->s_npcon}) {
     HIPCHK(b->alloc(n));
@@ -514,6 +514,7 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   S.gjac = s->s_gjac.p;
   S.grow = s->s_grow.p;
   S.isl_f = s->s_islf.p;
+  S.gmj = s->s_gmj.p;
   S.isl_i = s->s_isli.p;
 
   // invweight0 (forward::set_const) for the base variant, then poses (PippinCpu::new runs kinematics)

@@ -35,7 +35,7 @@ struct emu_sim {
   std::vector<uint32_t> v_ct, v_ca;
   std::vector<int> v_damped;
   std::vector<double> qpos, qvel, ctrl, qacc, applied, time, gxpos, gxmat, xpos, xmat, con, pcon, plambda, stage, gjac,
-      grow, islf;
+      grow, islf, gmj;
   std::vector<int> isli;
   std::vector<int> ncon, iters, nrows, status, cgeom, pcgeom, npcon;
   std::vector<double> lds;
@@ -171,6 +171,7 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   s->gjac.assign(nn * (int64_t)M.nrmax * d.nv, 0.0);
   s->grow.assign(nn * (int64_t)M.nrmax * 10, 0.0);
   s->islf.assign(std::max<int64_t>(nn * (int64_t)M.isl_fstride, 1), 0.0);
+  s->gmj.assign(M.solver_kind == 1 ? nn * (int64_t)M.nrmax * d.nv : 1, 0.0);
   s->isli.assign(std::max<int64_t>(nn * (int64_t)M.isl_istride, 1), 0);
   s->ncon.assign(nn, 0);
   s->iters.assign(nn, 0);
@@ -203,6 +204,7 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   S.gjac = s->gjac.data();
   S.grow = s->grow.data();
   S.isl_f = s->islf.data();
+  S.gmj = s->gmj.data();
   S.isl_i = s->isli.data();
   s->lds.assign(M.L.bytes / sizeof(double) + 16, 0.0);
   // set_const for the base variant

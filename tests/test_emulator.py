@@ -138,3 +138,53 @@ def test_multibody_islands_exact(assets, name):
         assert np.array_equal(o.get("qpos"), e.get("qpos")), t
     assert np.array_equal(o.solver_iterations(), e.solver_iterations())
     assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("name", ["plane_box", "box_box", "capsule_capsule", "plane_cylinder", "sphere_box"])
+def test_pgs_primitive_exact(name):
+    """PGS (solver.rs:44-289) on the kernel body against the oracle."""
+    arr = mjcf.load_string(primitive_xml(name))
+    arr["solver_kind"] = 1
+    n = 6
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(13)
+    q = o.get("qpos")
+    q[:, :3] += rng.normal(size=(n, 3)) * 0.02
+    v = rng.normal(size=(n, 6)) * 0.3
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+        s.set("qvel", 0, n, v)
+    for t in range(250):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert np.array_equal(o.get("qvel"), e.get("qvel"))
+
+
+@pytest.mark.parametrize("name", sorted(MULTIBODY_SCENES))
+def test_pgs_multibody_exact(assets, name):
+    xml = MULTIBODY_SCENES[name]
+    arr = mjcf.load_file(assets / "box_drop.xml") if xml is None else mjcf.load_string(xml)
+    arr["solver_kind"] = 1
+    n = 4
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    for t in range(300):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+
+
+def test_pgs_ant_exact(ant):
+    arr = dict(ant)
+    arr["solver_kind"] = 1
+    n = 8
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(1)
+    for t in range(150):
+        c = rng.uniform(-1, 1, size=(n, 8))
+        for s in (o, e):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t

@@ -176,3 +176,49 @@ def test_multibody_islands_match(assets, name):
         assert err <= 1e-9, (t, err)
     assert np.array_equal(o.get("qpos"), g.get("qpos"))
     assert np.array_equal(o.solver_iterations(), g.solver_iterations())
+
+
+@pytest.mark.parametrize("name", ["plane_box", "box_box", "capsule_capsule", "plane_cylinder"])
+def test_pgs_primitive_match(name):
+    """PGS solver (solver.rs:44-289) on the device against the oracle."""
+    arr = mjcf.load_string(primitive_xml(name))
+    arr["solver_kind"] = 1
+    n = 64
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(13)
+    q = o.get("qpos")
+    q[:, :3] += rng.normal(size=(n, 3)) * 0.02
+    v = rng.normal(size=(n, 6)) * 0.3
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+        s.set("qvel", 0, n, v)
+    say("pgs primitive", name)
+    for t in range(250):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        err = np.abs(o.get("qpos") - g.get("qpos")).max()
+        assert err <= 1e-9, (t, err)
+    assert np.array_equal(o.get("qpos"), g.get("qpos"))
+
+
+@pytest.mark.parametrize("name", sorted(MULTIBODY_SCENES))
+def test_pgs_multibody_match(assets, name):
+    xml = MULTIBODY_SCENES[name]
+    arr = mjcf.load_file(assets / "box_drop.xml") if xml is None else mjcf.load_string(xml)
+    arr["solver_kind"] = 1
+    n = 32
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(4)
+    nb = int(arr["nq"]) // 7
+    q = o.get("qpos")
+    for b in range(nb):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.01
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    say("pgs multibody", name)
+    for t in range(300):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+    assert np.array_equal(o.get("qpos"), g.get("qpos"))
