@@ -319,13 +319,25 @@ bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
   // candidate pairs in lexicographic order (model.rs:398-407)
   m.pair_g1.clear();
   m.pair_g2.clear();
+  m.general_g1.clear();
+  m.general_g2.clear();
   m.ncmax = 0;
   for (int g1 = 0; g1 < ng; g1++)
     for (int g2 = g1 + 1; g2 < ng; g2++) {
       if (!pair_static_ok(m, g1, g2)) continue;
       if (needs_general(m.geom_type[g1], m.geom_type[g2])) {
-        *err = "mesh and cylinder-vs-non-plane pairs need the general convex narrow phase, not implemented on the device";
-        return false;
+        // Mesh and cylinder-vs-non-plane pairs go through parry in the
+        // reference (collision.rs:102-147), which the device does not run.
+        // Such a pair is allowed only while the collision masks keep it off;
+        // it is listed so that a mask change enabling it can be refused.
+        bool on = (m.geom_contype[g1] & m.geom_conaffinity[g2]) || (m.geom_contype[g2] & m.geom_conaffinity[g1]);
+        if (on) {
+          *err = "mesh and cylinder-vs-non-plane pairs need the general convex narrow phase, not implemented on the device";
+          return false;
+        }
+        m.general_g1.push_back(g1);
+        m.general_g2.push_back(g2);
+        continue;
       }
       m.pair_g1.push_back(g1);
       m.pair_g2.push_back(g2);

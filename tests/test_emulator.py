@@ -71,15 +71,23 @@ def test_ant_trajectories_exact(ant):
     assert contacts > 0 and not e.status().any()
 
 
-@pytest.mark.parametrize("name,free", [("cartpole.xml", False), ("pendulum.xml", False)])
-def test_contact_free_models_exact(assets, name, free):
+@pytest.mark.parametrize("name", ["cartpole.xml", "pendulum.xml", "tumble.xml", "arm3d.xml"])
+def test_contact_free_models_exact(assets, name):
+    """Contact-free models: hinge/slide chains, a free body with a hinged
+    appendage, and arm3d (ball and slide joints, motor/position/velocity
+    actuators, damping, armature, general pairs masked off)."""
     arr = mjcf.load_file(assets / name)
     n = 8
     o, e = OracleSim(arr, n), EmuSim(arr, n)
     rng = np.random.default_rng(5)
-    q = _perturbed(o.get("qpos"), rng, 0.2, free)
-    for s in (o, e):
-        s.set("qpos", 0, n, q)
+    if name in ("tumble.xml", "arm3d.xml"):
+        v = rng.normal(size=(n, int(arr["nv"])))
+        for s in (o, e):
+            s.set("qvel", 0, n, v)
+    else:
+        q = _perturbed(o.get("qpos"), rng, 0.2, False)
+        for s in (o, e):
+            s.set("qpos", 0, n, q)
     nu = int(arr["nu"])
     for t in range(200):
         c = rng.uniform(-1, 1, size=(n, nu))

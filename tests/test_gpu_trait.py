@@ -172,16 +172,25 @@ def test_rollout_device_matches_oracle_rollout(assets, name):
     assert np.array_equal(g.get("qpos"), o.get("qpos"))
 
 
-@pytest.mark.parametrize("name", ["cartpole.xml", "pendulum.xml"])
+@pytest.mark.parametrize("name", ["cartpole.xml", "pendulum.xml", "tumble.xml", "arm3d.xml"])
 def test_contact_free_trajectories(assets, name):
-    """Contact-free bar: qpos within 1e-12 over 200 steps (held to exact equality)."""
+    """Contact-free bar: qpos within 1e-12 over 200 steps (held to exact equality).
+
+    tumble (free body + hinge) and arm3d (hinge, ball, slide; motor, position
+    and velocity actuators; joint damping and armature) start from random
+    velocities, the others from perturbed positions."""
     arr = mjcf.load_file(assets / name)
     n = 64
     o, g = _pair(arr, n)
     rng = np.random.default_rng(2)
-    q = o.get("qpos") + rng.normal(size=(n, int(arr["nq"]))) * 0.2
-    for s in (o, g):
-        s.set("qpos", 0, n, q)
+    if name in ("tumble.xml", "arm3d.xml"):
+        v = rng.normal(size=(n, int(arr["nv"])))
+        for s in (o, g):
+            s.set("qvel", 0, n, v)
+    else:
+        q = o.get("qpos") + rng.normal(size=(n, int(arr["nq"]))) * 0.2
+        for s in (o, g):
+            s.set("qpos", 0, n, q)
     nu = int(arr["nu"])
     for t in range(200):
         c = rng.uniform(-1, 1, size=(n, nu))
@@ -193,3 +202,16 @@ def test_contact_free_trajectories(assets, name):
     say(name, "200 steps max qpos err", err)
     assert err <= 1e-12
     assert err == 0.0
+
+
+def test_masks_cannot_enable_general_pairs(assets):
+    """arm3d's cylinder meets boxes and capsules only through the general convex
+    narrow phase, which the device does not run: the model is accepted while
+    its collision masks keep those pairs off, and a mask change that would turn
+    one on is refused without touching the stored masks."""
+    g = HipPhysics(mjcf.load_file(assets / "arm3d.xml"), 3)
+    g.set_param("geom_contype", 0, 0, 3, [1.0, 1.0, 1.0])  # no conaffinity anywhere: still off
+    with pytest.raises(ParamError, match="general"):
+        g.set_param("geom_conaffinity", 2, 1, 1, [1.0])  # cylinder (geom 0) x box (geom 2)
+    assert np.array_equal(g.get_param("geom_conaffinity", 2)[:, 0], [0, 0, 0])
+    assert np.array_equal(g.get_param("geom_contype", 0)[:, 0], [1, 1, 1])
