@@ -212,6 +212,6 @@ def test_masks_cannot_enable_general_pairs(assets):
     g = HipPhysics(mjcf.load_file(assets / "arm3d.xml"), 3)
     g.set_param("geom_contype", 0, 0, 3, [1.0, 1.0, 1.0])  # no conaffinity anywhere: still off
     with pytest.raises(ParamError, match="general"):
-        g.set_param("geom_conaffinity", 2, 1, 1, [1.0])  # cylinder (geom 0) x box (geom 2)
+        g.set_param("geom_conaffinity", 2, 1, 2, [1.0])  # cylinder (geom 0) x box (geom 2)
     assert np.array_equal(g.get_param("geom_conaffinity", 2)[:, 0], [0, 0, 0])
     assert np.array_equal(g.get_param("geom_contype", 0)[:, 0], [1, 1, 1])
