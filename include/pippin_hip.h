@@ -240,6 +240,45 @@ int pippin_hip_kernel_timing(ph_sim* s, int32_t reset, double* total_ms, int64_t
 /* Stream the library launches on (a hipStream_t). */
 void* pippin_hip_stream(ph_sim* s);
 
+/* ---- learned Newton warm start ----
+ * No counterpart in the reference: its warm start is the previous acceleration,
+ * taken when its cost is lower than that of a0 (crates/pippin/src/newton.rs:238-245).
+ * A 24 -> H -> H -> H -> 4 GELU MLP (BF16 weights, FP32 accumulation on the
+ * matrix cores) predicts the four pyramid-row forces of each of an env's first 16
+ * touching contacts; a0 + M^-1 J^T lambda becomes a third candidate, used only
+ * if its Newton cost is lower than the reference's choice. Newton solver only. */
+#define PH_MLP_LOG 1  /* record features and solved forces of every step */
+#define PH_MLP_WARM 2 /* use the predicted candidate (hidden width <= 64) */
+/* Allocate the network (hidden width H, a multiple of 16 up to 256; weights
+ * U(+-1/sqrt(fan_in)) from `seed`), the ring of recorded contacts (`capacity`
+ * samples, at least envs x 16) and the training scratch (batches up to max_batch,
+ * a multiple of 16). Mode starts at 0 (off). */
+int pippin_hip_mlp_init(ph_sim* s, int32_t hidden, uint64_t seed, int64_t capacity, int64_t max_batch);
+/* PH_MLP_LOG | PH_MLP_WARM, or 0: steps run the reference's path unchanged. */
+int pippin_hip_mlp_mode(ph_sim* s, int32_t mode);
+/* Append the contacts recorded by the last logged launch to the ring (env order);
+ * *stored = samples held (at most capacity; oldest overwritten). */
+int pippin_hip_mlp_collect(ph_sim* s, int64_t* stored);
+/* `iters` Adam steps (lr, betas 0.9 / 0.999, eps 1e-8) on batches of `batch` samples
+ * drawn from the ring by a hash of (seed, step); loss_out[iters] (optional) gets each
+ * batch's mean squared error on log1p(lambda / D). */
+int pippin_hip_mlp_train(ph_sim* s, int32_t iters, int64_t batch, float lr, uint64_t seed, float* loss_out);
+/* Parameter count, -1 before mlp_init. Layout: W1 [32][H] b1 [H] W2 [H][H] b2 W3 [H][H] b3
+ * W4 [H][16] b4 [16] (FP32, row-major [in][out]; input rows 24..31, output columns 4..15
+ * and b4[4..15] are padding and must stay 0). */
+int64_t pippin_hip_mlp_num_params(const ph_sim* s);
+int pippin_hip_mlp_get_params(ph_sim* s, float* out, int64_t n);
+/* Replace the parameters (restarts Adam). */
+int pippin_hip_mlp_set_params(ph_sim* s, const float* in, int64_t n);
+/* Host arrays: x (n x 24 features) -> y (n x 4 outputs); n a multiple of 16. */
+int pippin_hip_mlp_forward(ph_sim* s, const float* x, int64_t n, float* y);
+/* Mean squared error of the outputs against t (n x 4) and its gradient w.r.t. the
+ * parameters (num_params floats); no update. */
+int pippin_hip_mlp_loss_grad(ph_sim* s, const float* x, const float* t, int64_t n, float* grad, float* loss);
+/* Records of env `env` from the last step: features (16 x 24), predictions and
+ * targets (16 x 4 each, log1p(lambda / D)) and the number of contacts recorded. */
+int pippin_hip_mlp_records(ph_sim* s, int64_t env, float* feat, float* pred, float* tgt, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

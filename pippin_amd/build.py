@@ -14,7 +14,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libpippin_hip.so"
-SOURCES = ["runtime.hip", "host_model.cpp", "mjcf.cpp"]
+SOURCES = ["runtime.hip", "mlp.hip", "host_model.cpp", "mjcf.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PIPPIN_HIP_ARCH", "gfx950")
 FLAGS = [
@@ -49,14 +49,18 @@ def build(verbose: bool = False) -> Path:
     bdir = HERE / "_build"
     bdir.mkdir(exist_ok=True)
     generate_specs(bdir, verbose)
-    for src in SOURCES:
+    procs = []
+    for src in SOURCES:  # compiled side by side
         obj = bdir / (src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
         cmd = [HIPCC, *FLAGS, *lang, "-I", str(bdir), "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
         objs.append(str(obj))
+    failed = [cmd for cmd, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [HIPCC, *FLAGS, "-shared", "-o", str(tmp), *objs]
     subprocess.run(cmd, check=True)

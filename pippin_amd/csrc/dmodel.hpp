@@ -8,6 +8,7 @@
 #include <cstring>
 
 #include "dmath.hpp"
+#include "mlp.hpp"
 
 namespace phk {
 
@@ -130,6 +131,7 @@ struct DModel {
   gp<const int> v_damped;
 
   Layout L;
+  MlpW mlp;  // learned warm start (mode 0: off)
 };
 
 // Per-env state and scratch in global memory; env e owns slice e of each.
@@ -150,6 +152,11 @@ struct DState {
   gp<Real> gmj;       // n x nrmax x nv: PGS M^-1 J^T rows
   gp<Real> isl_f;     // n x isl_fstride: gathered island problems
   gp<int> isl_i;      // n x isl_istride: row block masks, gathered indices, islands
+  // learned warm start: per env, the first MLP_CL touching contacts of the last step
+  gp<float> mlp_feat;  // n x MLP_CL x MLP_NF features
+  gp<float> mlp_pred;  // n x MLP_CL x MLP_NO predicted log1p(lambda / D)
+  gp<float> mlp_tgt;   // n x MLP_CL x MLP_NO solved log1p(lambda / D) (logging on)
+  gp<int> mlp_cnt;     // n: contacts recorded
 };
 
 }  // namespace phk

@@ -402,7 +402,7 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
   if (max_contacts > 0) M.ncmax = std::min(M.ncmax, max_contacts);
   M.nrmax = std::max(4 * M.ncmax + 2 * hm.nlimited + d.neq, 1);
   // island scratch (newton.rs:378-473), only when several dof blocks exist
-  M.isl_fstride = M.nblock > 1 ? d.nv * d.nv + M.nrmax * (d.nv + 2) + 3 * d.nv : 0;
+  M.isl_fstride = M.nblock > 1 ? d.nv * d.nv + M.nrmax * (d.nv + 2) + 4 * d.nv : 0;
   M.isl_istride = M.nblock > 1 ? 2 * M.nrmax + d.nv + 2 * M.nblock + 1 : 0;
   // Rows beyond nrl live in HBM. Keep up to 32 in LDS, fewer if that keeps
   // the per-env working set within 10 KiB (four 4-env workgroups per CU).

@@ -792,6 +792,7 @@ struct NP {
   Real* a;                             // in: unconstrained acceleration; out: solution
   Real* qf;                            // out: constraint force, nv
   Real* fac;                           // nv x nv factor storage
+  const Real* warm2;                   // learned warm-start candidate, or null
 };
 
 template <class X>
@@ -904,6 +905,15 @@ PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
     });
     Real c0 = cost(a0, a0), c1 = cost(warm, a0);
     if (c1 < c0) a = warm;
+    if (P.warm2) {
+      // the learned candidate replaces the reference's choice only when cheaper
+      const LV w2 = X::gen([&](int l) {
+        Real v = P.warm2[clampi(l, nv)];
+        return l < nv ? v : 0.0;
+      });
+      Real c2 = cost(w2, a0);
+      if (c2 < (c1 < c0 ? c1 : c0)) a = w2;
+    }
   }
   const LV ma0 = mul_m(a0);
   const Real scale = 1.0 + sqrt(X::uni(lsum_lanes<X>(ma0 * ma0, nv, -0.0)));
@@ -1042,6 +1052,11 @@ PHD int newton_dense(const MT& M, Env& E, const NP& P) {
   cost_into(a0, 0);
   cost_into(warm, 1);
   if (sc[1] < sc[0]) X::par(nv, [&](int k) { a[k] = warm[k]; });
+  if (P.warm2) {
+    // the learned candidate replaces the reference's choice only when cheaper
+    cost_into(P.warm2, 4);
+    if (sc[4] < (sc[1] < sc[0] ? sc[1] : sc[0])) X::par(nv, [&](int k) { a[k] = P.warm2[k]; });
+  }
   mul_m<X>(P, a0, tmp);
   X::par(1, [&](int) {
     Real s = rsum0();
@@ -1167,15 +1182,15 @@ PHD int newton_dense(const MT& M, Env& E, const NP& P) {
 // is gathered (its dofs block by block, its rows in order), solved, and
 // scattered back. Iterations: the maximum over islands.
 template <class X, class MT>
-PHD int newton_solve(const MT& M, const DState& S, Env& E, int nr) {
+PHD int newton_solve(const MT& M, const DState& S, Env& E, int nr, const Real* warm2) {
   const int nv = M.nv;
   const int n_ineq = nr - M.neq;
   auto whole = [&]() {
     if (nv <= GL && nr <= GL && nr <= M.nrl) {
-      NP P{nv, nr, n_ineq, LD(qm), LD(J), LD(aref), LD(dd), LD(qacc), LD(a), LD(qfrc_c), LD(chol)};
+      NP P{nv, nr, n_ineq, LD(qm), LD(J), LD(aref), LD(dd), LD(qacc), LD(a), LD(qfrc_c), LD(chol), warm2};
       return newton_lanes<X>(M, E, P);
     }
-    NP P{nv, nr, n_ineq, LD(qm), E.J, E.aref, E.dd, LD(qacc), LD(a), LD(qfrc_c), LD(chol)};
+    NP P{nv, nr, n_ineq, LD(qm), E.J, E.aref, E.dd, LD(qacc), LD(a), LD(qfrc_c), LD(chol), warm2};
     return newton_dense<X>(M, E, P);
   };
   if (M.nblock <= 1) return whole();
@@ -1194,6 +1209,7 @@ PHD int newton_solve(const MT& M, const DState& S, Env& E, int nr) {
   Real* sa = sdd + M.nrmax;
   Real* swarm = sa + nv;
   Real* sqf = swarm + nv;
+  Real* swarm2 = sqf + nv;
   int* isc = E.si + M.L.i_iscal;
   const Real* J = E.J;
   X::par(nr, [&](int r) {
@@ -1260,8 +1276,9 @@ PHD int newton_solve(const MT& M, const DState& S, Env& E, int nr) {
     X::par(k, [&](int t) {
       sa[t] = LD(a)[sidx[t]];
       swarm[t] = LD(qacc)[sidx[t]];
+      if (warm2) swarm2[t] = warm2[sidx[t]];
     });
-    NP P{k, q, qi, sqm, sJ, saref, sdd, swarm, sa, sqf, LD(chol)};
+    NP P{k, q, qi, sqm, sJ, saref, sdd, swarm, sa, sqf, LD(chol), warm2 ? swarm2 : nullptr};
     int it = (k <= GL && q <= GL) ? newton_lanes<X>(M, E, P) : newton_dense<X>(M, E, P);
     iters = it > iters ? it : iters;
     X::par(k, [&](int t) {
@@ -1553,9 +1570,14 @@ PHD void integrate_pos(const MT& M, Env& E, const Real* v, Real h) {
 }
 
 // ---------------- one step, forward.rs:336-389 (Newton) ----------------
-// Returns false when the env cannot be stepped (status set).
+enum { STEP_FAILED = -1, STEP_DONE = -2 };
+
+// The step up to the constraint rows: forward dynamics, collision, the
+// unconstrained acceleration a0 (in LD(a)) and the rows. Returns the row count,
+// STEP_FAILED when the env cannot be stepped (status set) or STEP_DONE when the
+// PGS branch has completed the step.
 template <class X, class MT>
-PHD bool step_env(const MT& M, const DState& S, Env& E) {
+PHD int step_front(const MT& M, const DState& S, Env& E) {
   const int nv = M.nv;
   const Real h = M.timestep;
   int* isc = E.si + M.L.i_iscal;
@@ -1564,9 +1586,8 @@ PHD bool step_env(const MT& M, const DState& S, Env& E) {
   velocity_rne<X>(M, E);
   passive_actuation<X>(M, E);
   collide<X>(M, S, E);
-  if (isc[7] & ST_CONTACT_OVERFLOW) return false;
+  if (isc[7] & ST_CONTACT_OVERFLOW) return STEP_FAILED;
   Real* smooth = LD(smooth);
-  Real* acc = LD(acc);
   Real* a = LD(a);
   X::par(nv, [&](int k) {
     Real s = LD(passive)[k] + LD(qfrc_act)[k] + LD(applied)[k] - LD(bias)[k];
@@ -1582,31 +1603,44 @@ PHD bool step_env(const MT& M, const DState& S, Env& E) {
     // PGS on velocities; positions integrate with the biased velocity
     if (!factor_solve_mass<X>(M, E, h, a)) {
       X::par(1, [&](int) { isc[7] |= ST_NOTPD; });
-      return false;
+      return STEP_FAILED;
     }
     Real* vel = LD(vel);
     Real* vpos = LD(acc);
     X::par(nv, [&](int k) { vel[k] = LD(qvel)[k] + h * a[k]; });
     pgs_solve<X>(M, S, E, vel, vpos);
-    if (isc[7] & ST_ROW_OVERFLOW) return false;
+    if (isc[7] & ST_ROW_OVERFLOW) return STEP_FAILED;
     X::par(nv, [&](int k) { LD(qacc)[k] = (vel[k] - LD(qvel)[k]) / h; });
     integrate_pos<X>(M, E, vpos, h);
     X::par(nv, [&](int k) { LD(qvel)[k] = vel[k]; });
-    return true;
+    return STEP_DONE;
   }
   if (!factor_solve_mass<X>(M, E, 0.0, a)) {
     X::par(1, [&](int) { isc[7] |= ST_NOTPD; });
-    return false;
+    return STEP_FAILED;
   }
   X::par(nv, [&](int k) { LD(qfrc_c)[k] = 0.0; });
   int nr = build_rows<X>(M, S, E);
   if (nr < 0) {
     X::par(1, [&](int) { isc[7] |= ST_ROW_OVERFLOW; });
-    return false;
+    return STEP_FAILED;
   }
   X::par(1, [&](int) { isc[9] = nr; });
+  return nr;
+}
+
+// The rest of the Newton step: solve, damping, integration. warm2 is an extra
+// warm-start candidate (null: the reference's rule alone).
+template <class X, class MT>
+PHD bool step_back(const MT& M, const DState& S, Env& E, int nr, const Real* warm2) {
+  const int nv = M.nv;
+  const Real h = M.timestep;
+  int* isc = E.si + M.L.i_iscal;
+  Real* smooth = LD(smooth);
+  Real* acc = LD(acc);
+  Real* a = LD(a);
   if (nr > 0) {
-    int it = newton_solve<X>(M, S, E, nr);
+    int it = newton_solve<X>(M, S, E, nr, warm2);
     X::par(1, [&](int) { isc[8] = it; });
   }
   const bool damped = M.v_damped[E.var] != 0;
@@ -1629,6 +1663,14 @@ PHD bool step_env(const MT& M, const DState& S, Env& E) {
   integrate_pos<X>(M, E, vel, h);
   X::par(nv, [&](int k) { LD(qvel)[k] = vel[k]; });
   return true;
+}
+
+// Returns false when the env cannot be stepped (status set).
+template <class X, class MT>
+PHD bool step_env(const MT& M, const DState& S, Env& E) {
+  const int nr = step_front<X>(M, S, E);
+  if (nr < 0) return nr == STEP_DONE;
+  return step_back<X>(M, S, E, nr, nullptr);
 }
 
 // ---------------- set_const, forward.rs:275-333 ----------------
@@ -1758,6 +1800,149 @@ PHD void run_env(const MT& M, const DState& S, Env& E, int64_t nstep, const Real
     if (qpos_out) X::par(M.nq, [&](int k) { qpos_out[(rel * nstep + t) * M.nq + k] = lds[M.L.qpos + k]; });
   }
   // poses are not refreshed here: they are derived from qpos on demand (k_kin)
+  X::par(M.nq, [&](int k) { S.qpos[e * M.nq + k] = lds[M.L.qpos + k]; });
+  X::par(M.nv, [&](int k) {
+    S.qvel[e * M.nv + k] = lds[M.L.qvel + k];
+    S.qacc[e * M.nv + k] = lds[M.L.qacc + k];
+  });
+  X::par(M.nu, [&](int k) { S.ctrl[e * M.nu + k] = lds[M.L.ctrl + k]; });
+  X::par(1, [&](int) {
+    S.ncon[e] = isc[IS_NCON];
+    S.status[e] = isc[IS_STATUS];
+    S.iters[e] = isc[IS_ITERS];
+    S.nrows[e] = isc[IS_NROWS];
+    S.time[e] += (Real)nstep * M.timestep;
+  });
+}
+
+// ---------------- learned warm start (no reference counterpart) ----------------
+// Runs between the rows and Newton, on every env of a wavefront at once (the
+// network runs on the matrix cores over the wavefront's contacts): features of
+// the env's first MLP_CL touching contacts, the network (X::mlp_tiles), then
+// the candidate a0 + M^-1 J^T lambda = M^-1 (smooth + J^T lambda) in LD(vel),
+// which step_back reads before it reuses that array. code is the row count of
+// an env stepping now, STEP_FAILED / STEP_DONE for an env that is not, or
+// MLP_ABSENT for a lane group without an env (tail of the last workgroup).
+enum { MLP_ABSENT = -3 };
+
+template <class X, class MT>
+PHD const Real* mlp_stage(const MT& M, const DState& S, Env& E, int code) {
+  const int nv = M.nv;
+  int* isc = E.si + M.L.i_iscal;
+  const int64_t e = E.e;
+  int cnt = 0;
+  if (code > 0) {
+    const int na = isc[2];
+    cnt = na < MLP_CL ? na : MLP_CL;
+    const int nrows = code;
+    const Real* J = E.J;
+    const Real* aref = E.aref;
+    const Real* dd = E.dd;
+    const Real* a0 = LD(a);
+    const Real* warm = LD(qacc);
+    const Real* qv = LD(qvel);
+    const int* cact = reinterpret_cast<const int*>(LD(cact));
+    auto con = S.con + e * (int64_t)M.ncmax * CW;
+    auto cg = S.cgeom + e * (int64_t)M.ncmax * 3;
+    X::par(cnt, [&](int k) {
+      auto f = S.mlp_feat + (e * MLP_CL + k) * MLP_NF;
+      for (int j = 0; j < 4; j++) {
+        const int r = 4 * k + j;
+        const Real* jr = J + r * nv;
+        f[j] = (float)mlp_slog(sdot_n(jr, a0, nv) - aref[r]);      // residual at a0
+        f[4 + j] = (float)mlp_slog(sdot_n(jr, warm, nv) - aref[r]);  // at the previous acceleration
+        f[8 + j] = (float)mlp_slog(aref[r]);
+        f[12 + j] = (float)mlp_slog(sdot_n(jr, qv, nv));             // row velocity
+      }
+      const int c = cact[k];
+      auto cc = con + c * CW;
+      const int g1 = cg[3 * c], g2 = cg[3 * c + 1];
+      const Real mu = rmax(M.v_friction[(int64_t)E.var * M.ngeom + g1], M.v_friction[(int64_t)E.var * M.ngeom + g2]);
+      f[16] = (float)(0.1 * log(dd[4 * k]));
+      f[17] = (float)mlp_slog(1000.0 * cc[9]);  // depth in mm
+      f[18] = (float)mu;
+      f[19] = (float)cc[3];
+      f[20] = (float)cc[4];
+      f[21] = (float)cc[5];
+      f[22] = (float)(na / 16.0);
+      f[23] = (float)((nrows - 4 * na) / 16.0);  // limit and equality rows
+    });
+  }
+  X::par(1, [&](int) {
+    if (code != MLP_ABSENT) S.mlp_cnt[e] = cnt;
+    isc[13] = cnt;
+    isc[14] = 0;
+  });
+  X::mlp_tiles(M, S, e, isc[13]);
+  if (code <= 0 || cnt == 0 || !(M.mlp.mode & MLP_WARM)) return nullptr;
+  const Real* J = E.J;
+  const Real* dd = E.dd;
+  Real* w = LD(vel);
+  auto pred = S.mlp_pred + e * MLP_CL * MLP_NO;
+  X::par(nv, [&](int k) {
+    Real f = LD(smooth)[k];
+    for (int c = 0; c < cnt; c++)
+      for (int j = 0; j < MLP_NO; j++) {
+        const int r = 4 * c + j;
+        const Real z = (Real)pred[c * MLP_NO + j];
+        const Real lam = dd[r] * expm1(z > 0.0 ? z : 0.0);
+        f += lam * J[r * nv + k];
+      }
+    w[k] = f;
+  });
+  mass_solve<X>(M, E, w);
+  X::par(1, [&](int) {
+    bool ok = true;
+    for (int k = 0; k < nv; k++) ok = ok && rfinite(w[k]);
+    isc[14] = ok ? 1 : 0;
+  });
+  return isc[14] ? w : nullptr;
+}
+
+// Training targets after the solve: log1p(lambda_r / D_r) = log1p(max(0,
+// aref_r - J_r a)) for the rows of the recorded contacts.
+template <class X, class MT>
+PHD void mlp_targets(const MT& M, const DState& S, Env& E) {
+  const int nv = M.nv;
+  const int* isc = E.si + M.L.i_iscal;
+  const int cnt = isc[13];
+  const Real* J = E.J;
+  const Real* aref = E.aref;
+  const Real* a = LD(a);
+  auto tg = S.mlp_tgt + E.e * MLP_CL * MLP_NO;
+  X::par(cnt * MLP_NO, [&](int r) {
+    const Real v = aref[r] - sdot_n(J + r * nv, a, nv);
+    tg[r] = (float)log1p(v > 0.0 ? v : 0.0);
+  });
+}
+
+// run_env with the learned warm start. Every lane group of the wavefront
+// reaches mlp_stage once per step, whether its env steps, has failed or does
+// not exist, because the network runs on the whole wavefront.
+template <class X, class MT>
+PHD void run_env_mlp(const MT& M, const DState& S, Env& E, int64_t nstep, const Real* ctrl_seq, Real* qpos_out,
+                     int64_t rel, bool live) {
+  Real* lds = E.s;
+  int* isc = E.si + M.L.i_iscal;
+  const int64_t e = E.e;
+  if (live) load_state<X>(M, S, E);
+  for (int64_t t = 0; t < nstep; t++) {
+    const bool go = live && isc[IS_STATUS] == 0;
+    int code = live ? (int)STEP_FAILED : (int)MLP_ABSENT;
+    if (go) {
+      if (ctrl_seq) X::par(M.nu, [&](int k) { lds[M.L.ctrl + k] = ctrl_seq[(rel * nstep + t) * M.nu + k]; });
+      code = step_front<X>(M, S, E);
+    }
+    const Real* w2 = mlp_stage<X>(M, S, E, code);
+    bool ok = go && code == STEP_DONE;
+    if (code >= 0) {
+      ok = step_back<X>(M, S, E, code, w2);
+      if (ok && (M.mlp.mode & MLP_LOG)) mlp_targets<X>(M, S, E);
+      if (!ok) X::par(1, [&](int) { S.mlp_cnt[e] = 0; });
+    }
+    if (ok && qpos_out) X::par(M.nq, [&](int k) { qpos_out[(rel * nstep + t) * M.nq + k] = lds[M.L.qpos + k]; });
+  }
+  if (!live) return;
   X::par(M.nq, [&](int k) { S.qpos[e * M.nq + k] = lds[M.L.qpos + k]; });
   X::par(M.nv, [&](int k) {
     S.qvel[e * M.nv + k] = lds[M.L.qvel + k];

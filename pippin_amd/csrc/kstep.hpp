@@ -1,0 +1,111 @@
+// The step kernels: the execution policy for K envs per wavefront, the plain
+// step kernel and its variant with the learned warm start.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kbody.hpp"
+#include "mlp_dev.hpp"
+
+namespace phk {
+
+// K environments per workgroup of one 64-lane wavefront: env g of the
+// workgroup owns lanes [g * 64/K, (g + 1) * 64/K) and its own slice of the LDS.
+// Loops over an env's items stride over its lanes; the dense linear algebra
+// uses the first 16 of them (one DPP row).
+template <int K>
+struct DevXK : DevLanes {
+  static constexpr int G = 64 / K;
+  template <class F>
+  __device__ __forceinline__ static void par(int n, F&& f) {
+    for (int i = (int)(threadIdx.x % G); i < n; i += G) f(i);
+    __syncthreads();
+  }
+  __device__ __forceinline__ static void sync() { __syncthreads(); }
+};
+using DevX = DevXK<1>;
+
+// The policy of the learned-warm-start kernel: DevXK plus the network over
+// the wavefront's recorded contacts. Tile t holds contacts t * 16/K ..
+// (t + 1) * 16/K - 1 of each of the K envs, one sample column per contact;
+// tiles run while any env of the wavefront has contacts left.
+template <int K>
+struct DevXKM : DevXK<K> {
+  static constexpr int G = 64 / K;
+  template <class MT>
+  __device__ static void mlp_tiles(const MT& M, const DState& S, int64_t e_self, int cnt) {
+    constexpr int SPE = 16 / K;
+    if (M.mlp.h > MLP_HMAX_STEP) {  // wider networks are trained and run in batches only
+      __syncthreads();
+      return;
+    }
+    const int lane = (int)threadIdx.x, g = lane / G, j = lane & 15, q = lane >> 4;
+    const int64_t e0 = e_self - g;
+    int mx = cnt;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+    const int gs = j / SPE;  // env of this lane's sample column
+    const int cs = __shfl(cnt, gs * G);
+    const int nt = (mx + SPE - 1) / SPE;
+    const bf16x4 zero = {0, 0, 0, 0};
+    for (int t = 0; t < nt; t++) {
+      const int c = t * SPE + j % SPE;
+      const bool have = c < cs;
+      const int64_t row = (e0 + gs) * MLP_CL + c;
+      bf16x4 x0 = zero, x1 = zero;
+      if (have) {
+        const float* f = S.mlp_feat.p + row * MLP_NF;
+        x0 = pack_bf4(ld_f4(f + 4 * q));
+        if (q < 2) x1 = pack_bf4(ld_f4(f + 16 + 4 * q));
+      }
+      const f32x4 y = mlp_tile_fwd<MLP_HMAX_STEP / 16>(M.mlp, x0, x1, MlpNoSink{});
+      if (q == 0 && have) st_f4(S.mlp_pred.p + row * MLP_NO, y);
+    }
+    __syncthreads();
+  }
+};
+
+// Advance envs [begin, end) by nstep steps, K per workgroup. With ctrl_seq
+// (envs x nstep x nu) the controls change every step and qpos_out
+// (envs x nstep x nq) receives qpos after each step (Batch::rollout, batch.rs:46-59).
+// MT is DModel (sizes and layout read at run time) or a model-specialised
+// struct derived from it whose sizes and LDS layout are compile-time constants
+// (spec_models.hpp, generated at build time); both read the same buffers.
+template <int K, class MT>
+__global__ void __launch_bounds__(64) k_step(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                             int64_t begin, int64_t end, int64_t nstep, const Real* ctrl_seq,
+                                             Real* qpos_out) {
+  const MT& M = *static_cast<const MT*>(Mp);
+  const DState& S = *Sp;
+  extern __shared__ double lds[];
+  const int g = (int)threadIdx.x / DevXK<K>::G;
+  const int64_t rel = (int64_t)blockIdx.x * K + g;
+  if (begin + rel >= end) return;  // tail of the last workgroup
+  Env E;
+  E.s = lds + (int64_t)g * (M.L.bytes / 8);
+  E.si = reinterpret_cast<int*>(E.s + M.L.ndouble);
+  E.e = begin + rel;
+  E.var = M.env_var[E.e];
+  run_env<DevXK<K>>(M, S, E, nstep, ctrl_seq, qpos_out, rel);
+}
+
+// The step with the learned warm start (mlp.hpp). Lane groups past the end
+// stay in the loop without an env, since the network needs the whole wavefront.
+template <int K, class MT>
+__global__ void __launch_bounds__(64) k_step_mlp(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                                 int64_t begin, int64_t end, int64_t nstep, const Real* ctrl_seq,
+                                                 Real* qpos_out) {
+  const MT& M = *static_cast<const MT*>(Mp);
+  const DState& S = *Sp;
+  extern __shared__ double lds[];
+  const int g = (int)threadIdx.x / DevXK<K>::G;
+  const int64_t rel = (int64_t)blockIdx.x * K + g;
+  const bool live = begin + rel < end;
+  Env E;
+  E.s = lds + (int64_t)g * (M.L.bytes / 8);
+  E.si = reinterpret_cast<int*>(E.s + M.L.ndouble);
+  E.e = begin + rel;
+  E.var = live ? M.env_var[E.e] : 0;
+  run_env_mlp<DevXKM<K>>(M, S, E, nstep, ctrl_seq, qpos_out, rel, live);
+}
+
+}  // namespace phk

@@ -16,52 +16,13 @@
 
 #include "../../include/pippin_hip.h"
 #include "host_model.hpp"
-#include "kbody.hpp"
+#include "kstep.hpp"
+#include "mlp_host.hpp"
 #include "spec_models.hpp"
 
 using namespace phk;
 
 // ---------------------------------------------------------------- kernels
-// K environments per workgroup of one 64-lane wavefront: env g of the
-// workgroup owns lanes [g * 64/K, (g + 1) * 64/K) and its own slice of the LDS.
-// Loops over an env's items stride over its lanes; the dense linear algebra
-// uses the first 16 of them (one DPP row).
-template <int K>
-struct DevXK : DevLanes {
-  static constexpr int G = 64 / K;
-  template <class F>
-  __device__ __forceinline__ static void par(int n, F&& f) {
-    for (int i = (int)(threadIdx.x % G); i < n; i += G) f(i);
-    __syncthreads();
-  }
-  __device__ __forceinline__ static void sync() { __syncthreads(); }
-};
-using DevX = DevXK<1>;
-
-// Advance envs [begin, end) by nstep steps, K per workgroup. With ctrl_seq
-// (envs x nstep x nu) the controls change every step and qpos_out
-// (envs x nstep x nq) receives qpos after each step (Batch::rollout, batch.rs:46-59).
-// MT is DModel (sizes and layout read at run time) or a model-specialised
-// struct derived from it whose sizes and LDS layout are compile-time constants
-// (spec_models.hpp, generated at build time); both read the same buffers.
-template <int K, class MT>
-__global__ void __launch_bounds__(64) k_step(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
-                                             int64_t begin, int64_t end, int64_t nstep, const Real* ctrl_seq,
-                                             Real* qpos_out) {
-  const MT& M = *static_cast<const MT*>(Mp);
-  const DState& S = *Sp;
-  extern __shared__ double lds[];
-  const int g = (int)threadIdx.x / DevXK<K>::G;
-  const int64_t rel = (int64_t)blockIdx.x * K + g;
-  if (begin + rel >= end) return;  // tail of the last workgroup
-  Env E;
-  E.s = lds + (int64_t)g * (M.L.bytes / 8);
-  E.si = reinterpret_cast<int*>(E.s + M.L.ndouble);
-  E.e = begin + rel;
-  E.var = M.env_var[E.e];
-  run_env<DevXK<K>>(M, S, E, nstep, ctrl_seq, qpos_out, rel);
-}
-
 static const void* step_kernel(int K, int spec) {
   if (spec == 1 && K == 4) return (const void*)k_step<4, SpecAnt>;
   return K == 4 ? (const void*)k_step<4, DModel> : K == 2 ? (const void*)k_step<2, DModel> : (const void*)k_step<1, DModel>;
@@ -233,6 +194,17 @@ struct ph_sim {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double kernel_ms = 0.0;
   int64_t kernel_launches = 0;
+  // learned warm start (mlp.hpp): network, optimiser, batch scratch, ring of
+  // recorded contacts, per-env records of the last step
+  bool mlp_ready = false;
+  phm::Net net;
+  phm::Batch batch;
+  DBuf<float> m_P, m_m, m_v, m_G, m_z, m_y, m_loss, m_gp, m_rfeat, m_rtgt, m_feat, m_pred, m_tgt, m_xin, m_tin;
+  DBuf<uint16_t> m_wbf, m_wtbf, m_x0, m_a, m_dz;
+  DBuf<int> m_cnt;
+  DBuf<int64_t> m_total;
+  int64_t ring_cap = 0, ring_head = 0, ring_size = 0, adam_t = 0;
+  int64_t log_begin = 0, log_end = 0;  // envs recorded by the last launch with logging on
 };
 
 extern "C" const char* pippin_hip_last_error(void) { return g_err.c_str(); }
@@ -587,7 +559,13 @@ static void launch_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, co
   const int K = s->envs_per_wg;
   const unsigned grid = (unsigned)((end - begin + K - 1) / K);
   const size_t lds = (size_t)K * s->M.L.bytes;
-  if (s->spec == 1 && K == 4)
+  if (s->M.mlp.mode) {
+    phm::launch_step(K, s->spec, grid, lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep, ctrl, qpos_out);
+    if ((s->M.mlp.mode & MLP_LOG) && nstep > 0) {
+      s->log_begin = begin;
+      s->log_end = end;
+    }
+  } else if (s->spec == 1 && K == 4)
     hipLaunchKernelGGL((k_step<4, SpecAnt>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
                        end, nstep, ctrl, qpos_out);
   else if (K == 4)
@@ -1098,5 +1076,247 @@ extern "C" int pippin_hip_selftest_math(const double* x, int64_t n, double* out,
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(out, dout.p, 4 * n * sizeof(double), hipMemcpyDeviceToHost));
+  return PH_OK;
+}
+
+// ---------------------------------------------------------------- learned warm start
+// (no counterpart in the reference; the network and its use are described in mlp.hpp)
+
+static int mlp_check(ph_sim* s) {
+  if (!s) return fail(PH_ERR_ARG, "null sim");
+  if (!s->mlp_ready) return fail(PH_ERR_ARG, "learned warm start not initialised (pippin_hip_mlp_init)");
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_mlp_init(ph_sim* s, int32_t hidden, uint64_t seed, int64_t capacity, int64_t max_batch) {
+  if (!s) return fail(PH_ERR_ARG, "null sim");
+  if (s->M.solver_kind != 0) return fail(PH_ERR_ARG, "mlp_init: the learned warm start is for the Newton solver");
+  if (hidden < 16 || hidden > MLP_HMAX || hidden % 16)
+    return fail(PH_ERR_ARG, "mlp_init: hidden width must be a multiple of 16 in [16, 256]");
+  if (max_batch < 16 || max_batch % 16) return fail(PH_ERR_ARG, "mlp_init: max_batch must be a positive multiple of 16");
+  if (capacity < std::max<int64_t>(s->n * MLP_CL, max_batch))
+    return fail(PH_ERR_ARG, "mlp_init: capacity must hold one collect (envs x 16) and one batch");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const int h = hidden;
+  const int64_t np = mlp_nparam(h), nw = mlp_nweight(h), n = std::max<int64_t>(s->n, 1);
+  // weights ~ U(-1/sqrt(fan_in), 1/sqrt(fan_in)) over the real fan-in, padding and biases 0
+  std::vector<float> P(np, 0.0f);
+  for (int l = 0; l < 4; l++) {
+    const int IN = mlp_in(l, h), OUT = mlp_out(l, h);
+    const int fin = l == 0 ? MLP_NF : h;
+    const double bound = 1.0 / std::sqrt((double)fin);
+    for (int i = 0; i < IN; i++)
+      for (int o = 0; o < OUT; o++) {
+        if ((l == 0 && i >= MLP_NF) || (l == 3 && o >= MLP_NO)) continue;
+        const int64_t idx = mlp_woff(l, h) + (int64_t)i * OUT + o;
+        const double u = (double)(mlp_mix(seed + (uint64_t)idx) >> 11) * (1.0 / 9007199254740992.0);
+        P[idx] = (float)((2.0 * u - 1.0) * bound);
+      }
+  }
+  const int splits = 64;
+  HIPCHK(s->m_P.upload(P));
+  for (auto* b : {&s->m_m, &s->m_v, &s->m_G}) {
+    HIPCHK(b->alloc(np));
+    HIPCHK(b->zero());
+  }
+  HIPCHK(s->m_wbf.alloc(nw));
+  HIPCHK(s->m_wtbf.alloc(nw));
+  HIPCHK(s->m_x0.alloc(max_batch * MLP_KIN));
+  HIPCHK(s->m_z.alloc(3 * max_batch * h));
+  HIPCHK(s->m_a.alloc(3 * max_batch * h));
+  HIPCHK(s->m_dz.alloc(3 * max_batch * h + max_batch * MLP_NOP));
+  HIPCHK(s->m_y.alloc(max_batch * MLP_NOP));
+  HIPCHK(s->m_loss.alloc(max_batch / 16));
+  HIPCHK(s->m_gp.alloc(splits * np));
+  HIPCHK(s->m_xin.alloc(max_batch * MLP_NF));
+  HIPCHK(s->m_tin.alloc(max_batch * MLP_NO));
+  HIPCHK(s->m_rfeat.alloc(capacity * MLP_NF));
+  HIPCHK(s->m_rtgt.alloc(capacity * MLP_NO));
+  HIPCHK(s->m_feat.alloc(n * MLP_CL * MLP_NF));
+  HIPCHK(s->m_feat.zero());
+  HIPCHK(s->m_pred.alloc(n * MLP_CL * MLP_NO));
+  HIPCHK(s->m_pred.zero());
+  HIPCHK(s->m_tgt.alloc(n * MLP_CL * MLP_NO));
+  HIPCHK(s->m_tgt.zero());
+  HIPCHK(s->m_cnt.alloc(n));
+  HIPCHK(s->m_cnt.zero());
+  HIPCHK(s->m_total.alloc(1));
+  s->net = phm::Net{h, s->m_P.p, s->m_m.p, s->m_v.p, s->m_G.p, s->m_wbf.p, s->m_wtbf.p};
+  s->batch = phm::Batch{max_batch, splits, s->m_x0.p, s->m_z.p, s->m_a.p, s->m_dz.p, s->m_y.p, s->m_loss.p, s->m_gp.p};
+  phm::to_bf16(s->net, s->stream);
+  HIPCHK(hipGetLastError());
+  {
+    const int bytes = s->envs_per_wg * s->M.L.bytes;
+    if (bytes > 64 * 1024)
+      HIPCHK(hipFuncSetAttribute(phm::step_kernel(s->envs_per_wg, s->spec), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 bytes));
+  }
+  s->S.mlp_feat = s->m_feat.p;
+  s->S.mlp_pred = s->m_pred.p;
+  s->S.mlp_tgt = s->m_tgt.p;
+  s->S.mlp_cnt = s->m_cnt.p;
+  s->M.mlp = phm::view(s->net, 0);
+  s->ring_cap = capacity;
+  s->ring_head = s->ring_size = s->adam_t = 0;
+  s->log_begin = s->log_end = 0;
+  s->mlp_ready = true;
+  if (int rc = sync_desc(s)) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_mlp_mode(ph_sim* s, int32_t mode) {
+  if (int rc = mlp_check(s)) return rc;
+  if (mode & ~(MLP_LOG | MLP_WARM)) return fail(PH_ERR_ARG, "mlp_mode: unknown flag");
+  if ((mode & MLP_WARM) && s->net.h > MLP_HMAX_STEP)
+    return fail(PH_ERR_ARG, "mlp_mode: the step kernel runs networks up to 64 wide");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->M.mlp.mode = mode;
+  s->log_begin = s->log_end = 0;
+  return sync_desc(s);
+}
+
+extern "C" int pippin_hip_mlp_collect(ph_sim* s, int64_t* stored) {
+  if (int rc = mlp_check(s)) return rc;
+  HIPCHK(hipSetDevice(s->device));
+  int64_t total = 0;
+  if (s->log_end > s->log_begin) {
+    phm::collect(s->m_cnt.p, s->m_feat.p, s->m_tgt.p, s->log_begin, s->log_end, s->m_rfeat.p, s->m_rtgt.p,
+                 s->ring_cap, s->ring_head, s->m_total.p, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&total, s->m_total.p, sizeof(int64_t), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->ring_head = (s->ring_head + total) % s->ring_cap;
+    s->ring_size = std::min(s->ring_cap, s->ring_size + total);
+    s->log_begin = s->log_end = 0;  // each launch's records are appended once
+  }
+  if (stored) *stored = s->ring_size;
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_mlp_train(ph_sim* s, int32_t iters, int64_t batch, float lr, uint64_t seed,
+                                    float* loss_out) {
+  if (int rc = mlp_check(s)) return rc;
+  if (batch < 16 || batch % 16 || batch > s->batch.bmax)
+    return fail(PH_ERR_ARG, "mlp_train: batch must be a multiple of 16 up to max_batch");
+  if (s->ring_size == 0) return fail(PH_ERR_ARG, "mlp_train: no recorded contacts (log, step, collect first)");
+  if (!(lr > 0.0f)) return fail(PH_ERR_ARG, "mlp_train: learning rate must be positive");
+  HIPCHK(hipSetDevice(s->device));
+  std::vector<float> part(batch / 16);
+  for (int32_t it = 0; it < iters; it++) {
+    phm::forward_backward(s->net, s->batch, s->m_rfeat.p, s->m_rtgt.p, s->ring_size, batch, false, seed, s->adam_t,
+                          true, s->stream);
+    phm::adam(s->net, ++s->adam_t, lr, s->stream);
+    HIPCHK(hipGetLastError());
+    if (loss_out) {
+      HIPCHK(hipMemcpyAsync(part.data(), s->m_loss.p, part.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(hipStreamSynchronize(s->stream));
+      double sum = 0.0;
+      for (float x : part) sum += x;
+      loss_out[it] = (float)(sum / (4.0 * (double)batch));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PH_OK;
+}
+
+extern "C" int64_t pippin_hip_mlp_num_params(const ph_sim* s) {
+  if (!s || !s->mlp_ready) return -1;
+  return mlp_nparam(s->net.h);
+}
+
+extern "C" int pippin_hip_mlp_get_params(ph_sim* s, float* out, int64_t n) {
+  if (int rc = mlp_check(s)) return rc;
+  if (int rc = check_len("mlp_get_params", n, 1, mlp_nparam(s->net.h))) return rc;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(out, s->m_P.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return PH_OK;
+}
+
+// Replaces the parameters (padding entries must be 0) and restarts Adam.
+extern "C" int pippin_hip_mlp_set_params(ph_sim* s, const float* in, int64_t n) {
+  if (int rc = mlp_check(s)) return rc;
+  const int h = s->net.h;
+  if (int rc = check_len("mlp_set_params", n, 1, mlp_nparam(h))) return rc;
+  for (int64_t i = 0; i < n; i++)
+    if (!std::isfinite(in[i])) return fail(PH_ERR_ARG, "mlp_set_params: non-finite parameter");
+  for (int i = MLP_NF; i < MLP_KIN; i++)
+    for (int o = 0; o < h; o++)
+      if (in[mlp_woff(0, h) + (int64_t)i * h + o] != 0.0f) return fail(PH_ERR_ARG, "mlp_set_params: nonzero input padding");
+  for (int i = 0; i < h; i++)
+    for (int o = MLP_NO; o < MLP_NOP; o++)
+      if (in[mlp_woff(3, h) + (int64_t)i * MLP_NOP + o] != 0.0f)
+        return fail(PH_ERR_ARG, "mlp_set_params: nonzero output padding");
+  for (int o = MLP_NO; o < MLP_NOP; o++)
+    if (in[mlp_boff(3, h) + o] != 0.0f) return fail(PH_ERR_ARG, "mlp_set_params: nonzero output padding");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(s->m_P.p, in, n * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(s->m_m.zero());
+  HIPCHK(s->m_v.zero());
+  s->adam_t = 0;
+  phm::to_bf16(s->net, s->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PH_OK;
+}
+
+static int mlp_batch_in(ph_sim* s, const char* what, const float* x, const float* t, int64_t n) {
+  if (n < 16 || n % 16 || n > s->batch.bmax)
+    return fail(PH_ERR_ARG, std::string(what) + ": sample count must be a multiple of 16 up to max_batch");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(s->m_xin.p, x, n * MLP_NF * sizeof(float), hipMemcpyHostToDevice));
+  if (t) HIPCHK(hipMemcpy(s->m_tin.p, t, n * MLP_NO * sizeof(float), hipMemcpyHostToDevice));
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_mlp_forward(ph_sim* s, const float* x, int64_t n, float* y) {
+  if (int rc = mlp_check(s)) return rc;
+  if (int rc = mlp_batch_in(s, "mlp_forward", x, nullptr, n)) return rc;
+  phm::forward_backward(s->net, s->batch, s->m_xin.p, nullptr, n, n, true, 0, 0, false, s->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<float> yp(n * MLP_NOP);
+  HIPCHK(hipMemcpyAsync(yp.data(), s->m_y.p, yp.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int64_t i = 0; i < n; i++)
+    for (int o = 0; o < MLP_NO; o++) y[i * MLP_NO + o] = yp[i * MLP_NOP + o];
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_mlp_loss_grad(ph_sim* s, const float* x, const float* t, int64_t n, float* grad,
+                                        float* loss) {
+  if (int rc = mlp_check(s)) return rc;
+  if (int rc = mlp_batch_in(s, "mlp_loss_grad", x, t, n)) return rc;
+  phm::forward_backward(s->net, s->batch, s->m_xin.p, s->m_tin.p, n, n, true, 0, 0, true, s->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<float> part(n / 16);
+  HIPCHK(hipMemcpyAsync(part.data(), s->m_loss.p, part.size() * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  if (grad)
+    HIPCHK(hipMemcpyAsync(grad, s->m_G.p, mlp_nparam(s->net.h) * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  double sum = 0.0;
+  for (float v : part) sum += v;
+  if (loss) *loss = (float)(sum / (4.0 * (double)n));
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_mlp_records(ph_sim* s, int64_t env, float* feat, float* pred, float* tgt, int32_t* count) {
+  if (int rc = mlp_check(s)) return rc;
+  if (env < 0 || env >= s->n) return fail(PH_ERR_ARG, "mlp_records: env out of range");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  int c = 0;
+  HIPCHK(hipMemcpy(&c, s->m_cnt.p + env, sizeof(int), hipMemcpyDeviceToHost));
+  if (count) *count = c;
+  if (feat)
+    HIPCHK(hipMemcpy(feat, s->m_feat.p + env * MLP_CL * MLP_NF, MLP_CL * MLP_NF * sizeof(float), hipMemcpyDeviceToHost));
+  if (pred)
+    HIPCHK(hipMemcpy(pred, s->m_pred.p + env * MLP_CL * MLP_NO, MLP_CL * MLP_NO * sizeof(float), hipMemcpyDeviceToHost));
+  if (tgt)
+    HIPCHK(hipMemcpy(tgt, s->m_tgt.p + env * MLP_CL * MLP_NO, MLP_CL * MLP_NO * sizeof(float), hipMemcpyDeviceToHost));
   return PH_OK;
 }

@@ -27,6 +27,7 @@ SOLVER = {"newton": 0, "pgs": 1}
 PH_ERR_SIZE = -2
 
 _lib = None
+BOUND: list = []  # entry points given ctypes signatures
 
 
 class HipError(RuntimeError):
@@ -81,7 +82,19 @@ def lib():
             "pippin_hip_selftest_math": ([vp, i64, vp, i32], C.c_int),
             "pippin_hip_set_device": ([vp, i32, i64, i64, vp], C.c_int),
             "pippin_hip_kernel_timing": ([vp, i32, C.POINTER(C.c_double), C.POINTER(i64)], C.c_int),
+            "pippin_hip_mlp_init": ([vp, i32, C.c_uint64, i64, i64], C.c_int),
+            "pippin_hip_mlp_mode": ([vp, i32], C.c_int),
+            "pippin_hip_mlp_collect": ([vp, C.POINTER(i64)], C.c_int),
+            "pippin_hip_mlp_train": ([vp, i32, i64, C.c_float, C.c_uint64, vp], C.c_int),
+            "pippin_hip_mlp_num_params": ([vp], i64),
+            "pippin_hip_mlp_get_params": ([vp, vp, i64], C.c_int),
+            "pippin_hip_mlp_set_params": ([vp, vp, i64], C.c_int),
+            "pippin_hip_mlp_forward": ([vp, vp, i64, vp], C.c_int),
+            "pippin_hip_mlp_loss_grad": ([vp, vp, vp, i64, vp, vp], C.c_int),
+            "pippin_hip_mlp_records": ([vp, i64, vp, vp, vp, vp], C.c_int),
         }
+        global BOUND
+        BOUND = sorted(sigs)
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
             fn.argtypes = args
@@ -290,3 +303,56 @@ class HipPhysics:
         end = self.n if end is None else end
         self._check(self.L.pippin_hip_rollout_device(self.h, begin, end, nstep, C.c_void_p(ctrl_ptr),
                                                       C.c_void_p(qpos_out_ptr) if qpos_out_ptr else None))
+
+    # ---- learned Newton warm start (no counterpart in the reference; mlp.hpp) ----
+    MLP_LOG, MLP_WARM = 1, 2
+
+    def mlp_init(self, hidden=64, seed=0, capacity=None, max_batch=8192):
+        cap = capacity if capacity is not None else max(2_000_000, 16 * self.n, max_batch)
+        self._check(self.L.pippin_hip_mlp_init(self.h, hidden, seed, cap, max_batch))
+
+    def mlp_mode(self, mode: int):
+        self._check(self.L.pippin_hip_mlp_mode(self.h, mode))
+
+    def mlp_collect(self) -> int:
+        n = C.c_int64(0)
+        self._check(self.L.pippin_hip_mlp_collect(self.h, C.byref(n)))
+        return n.value
+
+    def mlp_train(self, iters, batch=4096, lr=1e-3, seed=0):
+        loss = np.zeros(iters, dtype=np.float32)
+        self._check(self.L.pippin_hip_mlp_train(self.h, iters, batch, lr, seed, _p(loss)))
+        return loss
+
+    def mlp_params(self):
+        n = self.L.pippin_hip_mlp_num_params(self.h)
+        out = np.zeros(n, dtype=np.float32)
+        self._check(self.L.pippin_hip_mlp_get_params(self.h, _p(out), n))
+        return out
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.L.pippin_hip_mlp_set_params(self.h, _p(p), p.size))
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], 4), dtype=np.float32)
+        self._check(self.L.pippin_hip_mlp_forward(self.h, _p(x), x.shape[0], _p(y)))
+        return y
+
+    def mlp_loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.L.pippin_hip_mlp_num_params(self.h), dtype=np.float32)
+        loss = C.c_float(0.0)
+        self._check(self.L.pippin_hip_mlp_loss_grad(self.h, _p(x), _p(t), x.shape[0], _p(g), C.byref(loss)))
+        return loss.value, g
+
+    def mlp_records(self, env):
+        f = np.zeros((16, 24), dtype=np.float32)
+        pr = np.zeros((16, 4), dtype=np.float32)
+        tg = np.zeros((16, 4), dtype=np.float32)
+        c = C.c_int32(0)
+        self._check(self.L.pippin_hip_mlp_records(self.h, env, _p(f), _p(pr), _p(tg), C.byref(c)))
+        k = c.value
+        return f[:k], pr[:k], tg[:k]

@@ -1,0 +1,175 @@
+"""The learned Newton warm start on the GPU (no counterpart in the reference).
+
+- The matrix-core network against the FP64 restatement (oracle/mlp.py) with the
+  same BF16 rounding points: outputs, loss and gradients of every layer.
+- With logging on and the candidate off, the warm-start kernel runs the
+  reference's path bit for bit (against the oracle), and its records are
+  consistent: the in-step predictions equal the batch forward of the recorded
+  features, the targets equal log1p(max(0, aref - J a)) >= 0.
+- A candidate that cannot win (all-zero network) leaves trajectories bit-exact.
+- Online training lowers the loss; the mean Newton iteration count with and
+  without the candidate is printed and must not grow.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mjcf
+from oracle import mlp as omlp
+from oracle.pyoracle import OracleSim
+from pippin_amd.physics import HipPhysics
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _ant(assets, n, seed=0):
+    arr = mjcf.load_file(assets / "ant.xml")
+    return arr
+
+
+@pytest.mark.parametrize("h", [64, 256])
+def test_network_matches_fp64_restatement(assets, h):
+    say("mlp forward/gradient, hidden", h)
+    g = HipPhysics(mjcf.load_file(assets / "ant.xml"), 16)
+    g.mlp_init(hidden=h, seed=3, max_batch=1024)
+    rng = np.random.default_rng(h)
+    P = omlp.random_params(h, rng, scale=1.5)
+    g.mlp_set_params(P)
+    assert np.array_equal(g.mlp_params(), P)
+    X = (rng.normal(size=(512, omlp.NF)) * 2.0).astype(np.float32)
+    T = rng.normal(size=(512, omlp.NO)).astype(np.float32)
+    y = g.mlp_forward(X)
+    y_ref, _ = omlp.forward(P, h, X, bf16=True)
+    y_exact, _ = omlp.forward(P, h, X, bf16=False)
+    say("  forward rel err vs bf16-rounded fp64", _rel(y, y_ref), "vs exact fp64", _rel(y, y_exact))
+    assert _rel(y, y_ref) < 2e-3
+    loss, G = g.mlp_loss_grad(X, T)
+    loss_ref, G_ref = omlp.loss_grad(P, h, X, T, bf16=True)
+    assert abs(loss - loss_ref) <= 2e-3 * loss_ref
+    for l in range(4):
+        for name, a, b in (("W", omlp.woff(l, h), omlp.boff(l, h)),
+                           ("b", omlp.boff(l, h), omlp.boff(l, h) + omlp.fan_out(l, h))):
+            e = _rel(G[a:b], G_ref[a:b])
+            say(f"  grad {name}{l + 1} rel err {e:.2e}")
+            assert e < 5e-3, (name, l, e)
+    # padding receives no gradient
+    W0 = omlp.unpack(G.astype(np.float64), h)[0]
+    assert not W0[0][omlp.NF:].any() and not W0[3][:, omlp.NO:].any()
+
+
+def _perturbed_ant(assets, n, seed):
+    arr = mjcf.load_file(assets / "ant.xml")
+    o = OracleSim(arr, n)
+    rng = np.random.default_rng(seed)
+    q = o.get("qpos") + rng.normal(size=(n, 15)) * 0.01
+    q[:, 3:7] /= np.linalg.norm(q[:, 3:7], axis=1, keepdims=True)
+    return arr, o, q, rng
+
+
+def test_logging_kernel_runs_the_reference_path_and_records(assets):
+    """n = 38: the last workgroup has 2 of its 4 envs (lane groups without an env)."""
+    n, T = 38, 60
+    arr, o, q, rng = _perturbed_ant(assets, n, 4)
+    g = HipPhysics(arr, n)
+    g.mlp_init(hidden=64, seed=1, max_batch=1024)
+    g.mlp_mode(HipPhysics.MLP_LOG)
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    recorded = 0
+    for t in range(T):
+        c = rng.uniform(-1, 1, size=(n, 8))
+        for s in (o, g):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+        if t % 10 == 9:
+            for e in range(0, n, 5):
+                f, pr, tg = g.mlp_records(e)
+                k = f.shape[0]
+                assert k == min(int((o.contacts(e)["depth"] > 0).sum()), 16)
+                recorded += k
+                if k:
+                    assert np.isfinite(f).all() and (tg >= 0).all()
+                    pad = np.zeros((16, 24), np.float32)
+                    pad[:k] = f
+                    y = g.mlp_forward(pad)[:k]
+                    assert np.allclose(pr, y, rtol=1e-6, atol=1e-6), (e, pr, y)
+    assert np.array_equal(o.solver_iterations(), g.solver_iterations())
+    assert recorded > 0
+    stored = g.mlp_collect()
+    assert stored == int(sum(min(int((o.contacts(e)["depth"] > 0).sum()), 16) for e in range(n)))
+
+
+def test_candidate_that_cannot_win_is_bit_exact(assets):
+    n, T = 40, 60
+    arr, o, q, rng = _perturbed_ant(assets, n, 5)
+    g = HipPhysics(arr, n)
+    g.mlp_init(hidden=64, seed=1, max_batch=1024)
+    g.mlp_set_params(np.zeros_like(g.mlp_params()))  # lambda = 0: the candidate is a0
+    g.mlp_mode(HipPhysics.MLP_LOG | HipPhysics.MLP_WARM)
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    for t in range(T):
+        c = rng.uniform(-1, 1, size=(n, 8))
+        for s in (o, g):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert np.array_equal(o.solver_iterations(), g.solver_iterations())
+
+
+def test_online_training_lowers_loss_and_iterations(assets):
+    """Ant, 2048 envs: record 30 steps, train, then compare mean Newton
+    iterations over the next 30 steps with and without the candidate, from
+    identical states."""
+    n = 2048
+    arr = mjcf.load_file(assets / "ant.xml")
+    rng = np.random.default_rng(7)
+    base = HipPhysics(arr, n)
+    g = HipPhysics(arr, n)
+    g.mlp_init(hidden=64, seed=2, max_batch=8192)
+    q = base.get("qpos") + rng.normal(size=(n, 15)) * 0.01
+    q[:, 3:7] /= np.linalg.norm(q[:, 3:7], axis=1, keepdims=True)
+    for s in (base, g):
+        s.set("qpos", 0, n, q)
+    ctrl = rng.uniform(-1, 1, size=(120, n, 8))
+    g.mlp_mode(HipPhysics.MLP_LOG)
+    stored = 0
+    for t in range(60):
+        for s in (base, g):
+            s.set("ctrl", 0, n, ctrl[t])
+            s.step(0, n, 1)
+        stored = g.mlp_collect()
+    assert np.array_equal(base.get("qpos"), g.get("qpos"))
+    loss = g.mlp_train(400, batch=4096, lr=2e-3, seed=1)
+    say("recorded", stored, "loss first", loss[:5].mean(), "last", loss[-20:].mean())
+    assert loss[-20:].mean() < 0.5 * loss[:5].mean()
+    g.mlp_mode(HipPhysics.MLP_WARM)
+    it_ref, it_mlp = [], []
+    for t in range(60, 90):
+        for s in (base, g):
+            s.set("ctrl", 0, n, ctrl[t])
+        # same state, both rules
+        g.set("qpos", 0, n, base.get("qpos"))
+        g.set("qvel", 0, n, base.get("qvel"))
+        base.step(0, n, 1)
+        g.step(0, n, 1)
+        it_ref.append(base.solver_iterations())
+        it_mlp.append(g.solver_iterations())
+        # one step from the same state: same solution up to the Newton tolerance
+        dq = np.abs(g.get("qvel") - base.get("qvel")).max()
+        assert dq < 1e-6, (t, dq)
+    a, b = np.mean(it_ref), np.mean(it_mlp)
+    say(f"mean Newton iterations: reference warm start {a:.3f}, with the learned candidate {b:.3f}"
+        f" ({100 * (a - b) / a:+.1f} %)")
+    assert b <= a * 1.02
