@@ -108,6 +108,68 @@ def copy_calibration(gib: float) -> None:
     print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "copies": 5}), flush=True)
 
 
+def warm_start_study(model: str, n: int, sub: int, record: int, train_iters: int, measure: int, seed: int):
+    """Mean Newton iterations per physics step with the reference warm start and
+    with the learned candidate (PH_MLP_WARM), from the same states, and the step
+    kernel time of each. The network is trained online on contacts recorded by
+    the same sim (PH_MLP_LOG) before the measurement."""
+    import torch
+    from pippin_amd.physics import HipPhysics, load_model
+
+    arrays = load_model(ROOT / "pippin_amd" / "assets" / model)
+    ref, mlp = HipPhysics(arrays, n), HipPhysics(arrays, n)
+    mlp.mlp_init(hidden=64, seed=seed, max_batch=8192)
+    rng = np.random.default_rng(seed)
+    q = ref.get("qpos")
+    nfree = [j for j in range(int(arrays["njnt"])) if int(np.asarray(arrays["jnt_type"])[j]) == 0]
+    if nfree and model == "ant.xml":
+        q = initial_qpos(np.asarray(arrays["qpos0"]), n, seed)
+    else:  # move each free body in the plane only
+        adr = np.asarray(arrays["jnt_qposadr"])
+        for j in nfree:
+            q[:, adr[j]:adr[j] + 2] += rng.normal(0.0, 0.01, size=(n, 2))
+    for s in (ref, mlp):
+        s.set("qpos", 0, n, q)
+    ctrl = rng.uniform(-1.0, 1.0, size=(record + measure, n, max(ref.nu, 1)))
+    mlp.mlp_mode(HipPhysics.MLP_LOG)
+    for a in range(record):
+        for s in (ref, mlp):
+            if ref.nu:
+                s.set("ctrl", 0, n, ctrl[a])
+            s.step(0, n, sub)
+        stored = mlp.mlp_collect()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = mlp.mlp_train(train_iters, batch=8192, lr=2e-3, seed=seed)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    it_ref, it_mlp = [], []
+    mlp.mlp_mode(HipPhysics.MLP_WARM)
+    ref.kernel_timing(reset=True)
+    mlp.kernel_timing(reset=True)
+    for a in range(record, record + measure):
+        mlp.set("qpos", 0, n, ref.get("qpos"))
+        mlp.set("qvel", 0, n, ref.get("qvel"))
+        for s in (ref, mlp):
+            if ref.nu:
+                s.set("ctrl", 0, n, ctrl[a])
+        for _ in range(sub):
+            ref.step(0, n, 1)
+            mlp.step(0, n, 1)
+            it_ref.append(ref.solver_iterations())
+            it_mlp.append(mlp.solver_iterations())
+    kr, lr_ = ref.kernel_timing(reset=True)
+    km, lm = mlp.kernel_timing(reset=True)
+    a, b = np.concatenate(it_ref), np.concatenate(it_mlp)
+    stats = lambda x: {"mean": float(x.mean()), "p50": float(np.percentile(x, 50)),
+                       "p99": float(np.percentile(x, 99))}
+    return {"model": model, "envs": n, "recorded_contacts": int(stored), "train_steps": train_iters,
+            "train_s": train_s, "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-10:].mean()),
+            "iters_reference": stats(a), "iters_learned": stats(b),
+            "iters_reduction": float((a.mean() - b.mean()) / a.mean()),
+            "kernel_ms_per_step_reference": kr / lr_, "kernel_ms_per_step_learned": km / lm}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +186,16 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device-to-device copy of this many GiB five times "
                          "(reference kernel for scaling HBM byte counters), then exit")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="only run the learned warm-start study (Ant and box_drop: mean Newton iterations "
+                         "with and without the candidate) and print one JSON line per model")
     args = ap.parse_args()
+    if args.warm_start:
+        for model, n, sub, rec, meas in (("ant.xml", args.envs, args.substeps, 100, 25),
+                                          ("box_drop.xml", args.envs, 1, 400, 100)):
+            log(f"warm-start study: {model}, {n} envs")
+            print(json.dumps(warm_start_study(model, n, sub, rec, 1500, meas, args.seed)), flush=True)
+        return
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
         return

@@ -37,6 +37,7 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     # and the Python front end binds exactly what the header declares
     physics.lib()
+    assert physics.BOUND == declared_functions()
 
 
 @pytest.mark.parametrize("name", ["ant.xml", "cartpole.xml", "pendulum.xml", "box_drop.xml", "tumble.xml",
