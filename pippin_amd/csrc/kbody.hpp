@@ -921,11 +921,18 @@ PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
   const int cap = M.iterations > 1 ? M.iterations : 1;
   uint64_t prev = ~(uint64_t)0;
   int iters = 0;
+  // residuals and active set at the current a, kept for the final force while a is unchanged
+  LV jar_a = X::splat(0.0);
+  uint32_t act_a = 0;
+  bool fresh = false;
   while (iters < cap) {
     iters++;
     const LV tmp = a - a0;
     const LV jar = jdot(a) - aref;
     const uint32_t act = active_mask(jar);
+    jar_a = jar;
+    act_a = act;
+    fresh = true;
     const LV grad = add_rows(mul_m(tmp), dd * jar, act);
     const Real gnorm = sqrt(X::uni(lsum_lanes<X>(grad * grad, nv, -0.0)));
     if (gnorm < tol * scale || ((uint64_t)act == prev && iters > 1)) break;
@@ -996,10 +1003,14 @@ PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
       al = next;
     }
     a = a + al * p;
+    fresh = false;
   }
-  // add_constraint_force (newton.rs:361-371), from zero
-  const LV jar = jdot(a) - aref;
-  const LV f = add_rows(X::splat(0.0), (-dd) * jar, active_mask(jar));
+  // add_constraint_force (newton.rs:361-371), from zero, at the final a
+  if (!fresh) {
+    jar_a = jdot(a) - aref;
+    act_a = active_mask(jar_a);
+  }
+  const LV f = add_rows(X::splat(0.0), (-dd) * jar_a, act_a);
   X::each([&](int l) {
     if (l < nv) {
       P.a[l] = X::at(a, l);
