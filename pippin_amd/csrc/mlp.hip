@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(64) k_mlp_tile(MlpW W, TileArgs T) {
 
 // dW_l = A_{l-1}^T dZ_l over one split of the batch, one 16 x 16 block per
 // wavefront: A operand rows = inputs, k = samples; B operand k = samples,
-// columns = outputs.
+// columns = outputs. Bias blocks (after the weight blocks) use an A operand
+// whose row 0 is all ones, so that row 0 of the result is db_l = 1^T dZ_l.
 struct WgArgs {
   const uint16_t* x0;
   const uint16_t* a;
@@ -130,7 +131,15 @@ __global__ void __launch_bounds__(64) k_mlp_wgrad(int h, WgArgs A) {
   const int lane = (int)threadIdx.x, j = lane & 15, q = lane >> 4;
   const int nb = h >> 4;
   int blk = (int)blockIdx.x, l, ib, ob;
-  if (blk < 2 * nb) {
+  bool bias = false;
+  const int nwblk = 2 * nb + 2 * nb * nb + nb;
+  if (blk >= nwblk) {  // bias blocks: layers 0..2 have nb each, layer 3 one
+    bias = true;
+    blk -= nwblk;
+    l = blk < 3 * nb ? blk / nb : 3;
+    ob = blk < 3 * nb ? blk % nb : 0;
+    ib = 0;
+  } else if (blk < 2 * nb) {
     l = 0;
     ib = blk / nb;
     ob = blk % nb;
@@ -155,30 +164,19 @@ __global__ void __launch_bounds__(64) k_mlp_wgrad(int h, WgArgs A) {
     bf16x4 av, bv;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      av[v] = (short)X[(s + 4 * q + v) * IN + 16 * ib + j];
+      av[v] = bias ? (short)(j == 0 ? 0x3f80 : 0) : (short)X[(s + 4 * q + v) * IN + 16 * ib + j];  // BF16 1.0
       bv[v] = (short)D[(s + 4 * q + v) * OUT + 16 * ob + j];
     }
     acc = mfma_bf16(av, bv, acc);
   }
-  float* out = A.gp + (int64_t)blockIdx.y * A.np + mlp_woff(l, h);
+  float* out = A.gp + (int64_t)blockIdx.y * A.np;
+  if (bias) {
+    if (q == 0) out[mlp_boff(l, h) + 16 * ob + j] = acc[0];  // row 0
+    return;
+  }
+  out += mlp_woff(l, h);
 #pragma unroll
   for (int v = 0; v < 4; v++) out[(int64_t)(16 * ib + 4 * q + v) * OUT + 16 * ob + j] = acc[v];
-}
-
-// db_l = column sums of dZ_l over one split.
-__global__ void __launch_bounds__(256) k_mlp_colsum(int h, const uint16_t* dz, float* gp, int64_t B, int splits,
-                                                    int64_t np) {
-  const int64_t chunk = B / splits, s0 = (int64_t)blockIdx.x * chunk;
-  const int total = 3 * h + MLP_NOP;
-  for (int o = (int)threadIdx.x; o < total; o += 256) {
-    const int l = o < 3 * h ? o / h : 3;
-    const int idx = o < 3 * h ? o % h : o - 3 * h;
-    const int OUT = mlp_out(l, h);
-    const uint16_t* D = dz + (int64_t)l * B * h;
-    float sum = 0.0f;
-    for (int64_t s = s0; s < s0 + chunk; s++) sum += bf2f(D[s * OUT + idx]);
-    gp[(int64_t)blockIdx.x * np + mlp_boff(l, h) + idx] = sum;
-  }
 }
 
 __global__ void __launch_bounds__(256) k_mlp_gsum(const float* gp, float* G, int64_t np, int splits) {
@@ -214,10 +212,10 @@ __global__ void __launch_bounds__(256) k_mlp_tobf16(int h, const float* P, uint1
   }
 }
 
-// Ring append: one workgroup scans the per-env counts in env order.
-__global__ void __launch_bounds__(1024) k_mlp_collect(const int* cnt, const float* feat, const float* tgt,
-                                                      int64_t begin, int64_t end, float* rfeat, float* rtgt,
-                                                      int64_t cap, int64_t head, int64_t* total) {
+// Ring append in env order: one workgroup scans the per-env counts into
+// offsets, then one thread per (env, contact) copies its record.
+__global__ void __launch_bounds__(1024) k_mlp_scan(const int* cnt, int64_t begin, int64_t end, int64_t* off,
+                                                   int64_t* total) {
   __shared__ int sc[1024];
   const int tid = (int)threadIdx.x;
   int64_t run = 0;
@@ -232,17 +230,24 @@ __global__ void __launch_bounds__(1024) k_mlp_collect(const int* cnt, const floa
       sc[tid] += add;
       __syncthreads();
     }
-    const int64_t off = run + sc[tid] - c;
-    for (int k = 0; k < c; k++) {
-      const int64_t dst = (head + off + k) % cap;
-      const int64_t srow = e * MLP_CL + k;
-      for (int i = 0; i < MLP_NF; i++) rfeat[dst * MLP_NF + i] = feat[srow * MLP_NF + i];
-      for (int i = 0; i < MLP_NO; i++) rtgt[dst * MLP_NO + i] = tgt[srow * MLP_NO + i];
-    }
+    if (e < end) off[e - begin] = run + sc[tid] - c;
     run += sc[1023];
     __syncthreads();
   }
   if (tid == 0) *total = run;
+}
+
+__global__ void __launch_bounds__(256) k_mlp_scatter(const int* cnt, const float* feat, const float* tgt,
+                                                     int64_t begin, int64_t end, const int64_t* off, float* rfeat,
+                                                     float* rtgt, int64_t cap, int64_t head) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t e = begin + t / MLP_CL;
+  const int k = (int)(t % MLP_CL);
+  if (e >= end || k >= cnt[e]) return;
+  const int64_t dst = (head + off[e - begin] + k) % cap;
+  const int64_t srow = e * MLP_CL + k;
+  for (int i = 0; i < MLP_NF; i++) rfeat[dst * MLP_NF + i] = feat[srow * MLP_NF + i];
+  for (int i = 0; i < MLP_NO; i++) rtgt[dst * MLP_NO + i] = tgt[srow * MLP_NO + i];
 }
 
 int pick_splits(int64_t B, int want) {
@@ -304,9 +309,8 @@ void forward_backward(const Net& n, const Batch& b, const float* feat, const flo
   const int splits = pick_splits(B, b.splits);
   const int nb = n.h / 16;
   WgArgs A{b.x0, b.a, b.dz, b.gp, B, np, splits};
-  hipLaunchKernelGGL(k_mlp_wgrad, dim3((unsigned)(2 * nb + 2 * nb * nb + nb), (unsigned)splits), dim3(64), 0, st,
-                     n.h, A);
-  hipLaunchKernelGGL(k_mlp_colsum, dim3((unsigned)splits), dim3(256), 0, st, n.h, b.dz, b.gp, B, splits, np);
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3((unsigned)(2 * nb + 2 * nb * nb + nb + 3 * nb + 1), (unsigned)splits), dim3(64),
+                     0, st, n.h, A);
   hipLaunchKernelGGL(k_mlp_gsum, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, b.gp, n.G, np, splits);
 }
 
@@ -320,9 +324,12 @@ void adam(const Net& n, int64_t t, float lr, hipStream_t st) {
 }
 
 void collect(const int* cnt, const float* feat, const float* tgt, int64_t begin, int64_t end, float* rfeat,
-             float* rtgt, int64_t cap, int64_t head, int64_t* d_total, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_collect, dim3(1), dim3(1024), 0, st, cnt, feat, tgt, begin, end, rfeat, rtgt, cap, head,
-                     d_total);
+             float* rtgt, int64_t cap, int64_t head, int64_t* d_scratch, hipStream_t st) {
+  // d_scratch: the total, then one offset per env
+  hipLaunchKernelGGL(k_mlp_scan, dim3(1), dim3(1024), 0, st, cnt, begin, end, d_scratch + 1, d_scratch);
+  const int64_t threads = (end - begin) * MLP_CL;
+  hipLaunchKernelGGL(k_mlp_scatter, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, cnt, feat, tgt, begin,
+                     end, d_scratch + 1, rfeat, rtgt, cap, head);
 }
 
 }  // namespace phm

@@ -50,9 +50,9 @@ void forward_backward(const Net& n, const Batch& b, const float* feat, const flo
 // One Adam step (t counts from 1), then the BF16 copies.
 void adam(const Net& n, int64_t t, float lr, hipStream_t st);
 // Append the recorded contacts of envs [begin, end) to the ring of capacity
-// cap at position head (env order, then contact order); *d_total receives
-// the number appended.
+// cap at position head (env order, then contact order). d_scratch holds
+// 1 + (end - begin) int64: d_scratch[0] receives the number appended.
 void collect(const int* cnt, const float* feat, const float* tgt, int64_t begin, int64_t end, float* rfeat,
-             float* rtgt, int64_t cap, int64_t head, int64_t* d_total, hipStream_t st);
+             float* rtgt, int64_t cap, int64_t head, int64_t* d_scratch, hipStream_t st);
 
 }  // namespace phm

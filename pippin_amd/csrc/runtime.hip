@@ -1141,7 +1141,7 @@ extern "C" int pippin_hip_mlp_init(ph_sim* s, int32_t hidden, uint64_t seed, int
   HIPCHK(s->m_tgt.zero());
   HIPCHK(s->m_cnt.alloc(n));
   HIPCHK(s->m_cnt.zero());
-  HIPCHK(s->m_total.alloc(1));
+  HIPCHK(s->m_total.alloc(1 + n));  // appended count, then per-env ring offsets
   s->net = phm::Net{h, s->m_P.p, s->m_m.p, s->m_v.p, s->m_G.p, s->m_wbf.p, s->m_wtbf.p};
   s->batch = phm::Batch{max_batch, splits, s->m_x0.p, s->m_z.p, s->m_a.p, s->m_dz.p, s->m_y.p, s->m_loss.p, s->m_gp.p};
   phm::to_bf16(s->net, s->stream);
