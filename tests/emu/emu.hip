@@ -7,6 +7,7 @@
 // device. It is never loaded by the product library.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -25,6 +26,41 @@ struct HostX : HostLanes {
   static void sync() {}
 };
 
+// The network of the learned warm start for one sample, in FP32 with BF16
+// rounding at the device's points (inputs, weights, hidden activations); the
+// sums run sequentially from the bias, where the device's matrix cores
+// accumulate in their own order.
+static void host_mlp_forward(const MlpW& W, const float* f, float* y) {
+  const int h = W.h;
+  float x[MLP_HMAX], a[MLP_HMAX];
+  for (int i = 0; i < MLP_KIN; i++) x[i] = i < MLP_NF ? bf2f(f2bf(f[i])) : 0.0f;
+  int nin = MLP_KIN;
+  for (int l = 0; l < 3; l++) {
+    for (int o = 0; o < h; o++) {
+      float acc = W.b[l][o];
+      for (int i = 0; i < nin; i++) acc += bf2f(W.wt[l][(int64_t)o * nin + i]) * x[i];
+      a[o] = bf2f(f2bf(mlp_gelu(acc)));
+    }
+    for (int o = 0; o < h; o++) x[o] = a[o];
+    nin = h;
+  }
+  for (int o = 0; o < MLP_NO; o++) {
+    float acc = W.b[3][o];
+    for (int i = 0; i < h; i++) acc += bf2f(W.wt[3][(int64_t)o * h + i]) * x[i];
+    y[o] = acc;
+  }
+}
+
+// The policy of the learned-warm-start kernel with one env at a time: the
+// network runs over this env's recorded contacts.
+struct HostXM : HostX {
+  template <class MT>
+  static void mlp_tiles(const MT& M, const DState& S, int64_t e, int cnt) {
+    for (int k = 0; k < cnt; k++)
+      host_mlp_forward(M.mlp, S.mlp_feat.p + (e * MLP_CL + k) * MLP_NF, S.mlp_pred.p + (e * MLP_CL + k) * MLP_NO);
+  }
+};
+
 struct emu_sim {
   phh::HostModel hm;
   DModel M;
@@ -39,6 +75,10 @@ struct emu_sim {
   std::vector<int> isli;
   std::vector<int> ncon, iters, nrows, status, cgeom, pcgeom, npcon;
   std::vector<double> lds;
+  // learned warm start
+  std::vector<float> mlp_P, mlp_feat, mlp_pred, mlp_tgt;
+  std::vector<uint16_t> mlp_wbf, mlp_wtbf;
+  std::vector<int> mlp_cnt;
 };
 
 static thread_local std::string g_err;
@@ -233,8 +273,55 @@ int emu_step(emu_sim* s, int64_t begin, int64_t end, int64_t nstep) {
   for (int64_t e = begin; e < end; e++) {
     E.e = e;
     E.var = s->env_var[e];
-    run_env<HostX>(s->M, s->S, E, nstep, nullptr, nullptr, e - begin);
+    if (s->M.mlp.mode)
+      run_env_mlp<HostXM>(s->M, s->S, E, nstep, nullptr, nullptr, e - begin, true);
+    else
+      run_env<HostX>(s->M, s->S, E, nstep, nullptr, nullptr, e - begin);
   }
+  return 0;
+}
+
+// Learned warm start: parameters in the device layout (mlp.hpp), hidden width h, mode.
+int emu_mlp_set(emu_sim* s, int h, int mode, const float* P) {
+  const int64_t np = mlp_nparam(h), nw = mlp_nweight(h), n = std::max<int64_t>(s->n, 1);
+  s->mlp_P.assign(P, P + np);
+  s->mlp_wbf.assign(nw, 0);
+  s->mlp_wtbf.assign(nw, 0);
+  for (int l = 0; l < 4; l++) {
+    const int IN = mlp_in(l, h), OUT = mlp_out(l, h);
+    for (int i = 0; i < IN; i++)
+      for (int o = 0; o < OUT; o++) {
+        const uint16_t b = f2bf(P[mlp_woff(l, h) + (int64_t)i * OUT + o]);
+        s->mlp_wbf[mlp_wbf_off(l, h) + (int64_t)i * OUT + o] = b;
+        s->mlp_wtbf[mlp_wbf_off(l, h) + (int64_t)o * IN + i] = b;
+      }
+  }
+  if (s->mlp_feat.empty()) {
+    s->mlp_feat.assign(n * MLP_CL * MLP_NF, 0.0f);
+    s->mlp_pred.assign(n * MLP_CL * MLP_NO, 0.0f);
+    s->mlp_tgt.assign(n * MLP_CL * MLP_NO, 0.0f);
+    s->mlp_cnt.assign(n, 0);
+  }
+  MlpW& W = s->M.mlp;
+  W.mode = mode;
+  W.h = h;
+  for (int l = 0; l < 4; l++) {
+    W.wt[l] = s->mlp_wtbf.data() + mlp_wbf_off(l, h);
+    W.w[l] = s->mlp_wbf.data() + mlp_wbf_off(l, h);
+    W.b[l] = s->mlp_P.data() + mlp_boff(l, h);
+  }
+  s->S.mlp_feat = s->mlp_feat.data();
+  s->S.mlp_pred = s->mlp_pred.data();
+  s->S.mlp_tgt = s->mlp_tgt.data();
+  s->S.mlp_cnt = s->mlp_cnt.data();
+  return 0;
+}
+
+int emu_mlp_records(emu_sim* s, int64_t env, float* feat, float* pred, float* tgt, int32_t* count) {
+  *count = s->mlp_cnt[env];
+  memcpy(feat, s->mlp_feat.data() + env * MLP_CL * MLP_NF, MLP_CL * MLP_NF * sizeof(float));
+  memcpy(pred, s->mlp_pred.data() + env * MLP_CL * MLP_NO, MLP_CL * MLP_NO * sizeof(float));
+  memcpy(tgt, s->mlp_tgt.data() + env * MLP_CL * MLP_NO, MLP_CL * MLP_NO * sizeof(float));
   return 0;
 }
 

@@ -31,7 +31,9 @@ def lib():
                      "emu_set": [vp, C.c_int, i64, i64, vp], "emu_ncon": [vp, i64, i64, vp],
                      "emu_iters": [vp, i64, i64, vp], "emu_status": [vp, i64, i64, vp],
                      "emu_contacts": [vp, i64, vp, C.c_int32, vp], "emu_mass_bias": [vp, i64, i64, vp, vp],
-                     "emu_invweight": [vp, vp, vp], "emu_poses": [vp, i64, i64, vp, vp]}.items():
+                     "emu_invweight": [vp, vp, vp], "emu_poses": [vp, i64, i64, vp, vp],
+                     "emu_mlp_set": [vp, C.c_int, C.c_int, vp],
+                     "emu_mlp_records": [vp, i64, vp, vp, vp, vp]}.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = C.c_int
         _lib = L
@@ -115,3 +117,17 @@ class EmuSim:
         gm = np.zeros((e - b, self.ngeom, 9))
         self.L.emu_poses(self.h, b, e, _p(gp), _p(gm))
         return gp, gm
+
+    # ---- learned warm start (the kernel's network stage, run on the host) ----
+    def mlp_set(self, h, mode, params):
+        self._mlp_params = np.ascontiguousarray(params, dtype=np.float32)
+        self.L.emu_mlp_set(self.h, h, mode, _p(self._mlp_params))
+
+    def mlp_records(self, env):
+        f = np.zeros((16, 24), dtype=np.float32)
+        pr = np.zeros((16, 4), dtype=np.float32)
+        tg = np.zeros((16, 4), dtype=np.float32)
+        c = C.c_int32(0)
+        self.L.emu_mlp_records(self.h, env, _p(f), _p(pr), _p(tg), C.byref(c))
+        k = c.value
+        return f[:k], pr[:k], tg[:k]

@@ -196,3 +196,69 @@ def test_pgs_ant_exact(ant):
             s.set("ctrl", 0, n, c)
             s.step(0, n, 1)
         assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+
+
+def _ant_pair(ant, n, seed):
+    o, e = OracleSim(ant, n), EmuSim(ant, n)
+    rng = np.random.default_rng(seed)
+    q = _perturbed(o.get("qpos"), rng, 0.01)
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+    return o, e, rng
+
+
+def test_learned_warm_start_logging_is_the_reference_path(ant):
+    """The learned-warm-start kernel body (run_env_mlp, mlp_stage, mlp_targets) on
+    the host with logging on: trajectories equal the oracle's bit for bit; the
+    records hold the touching contacts, predictions equal the FP64 restatement of
+    the network (oracle/mlp.py) on the recorded features, targets are
+    log1p(lambda / D) >= 0."""
+    from oracle import mlp as omlp
+
+    n, h = 8, 32
+    o, e, rng = _ant_pair(ant, n, 21)
+    P = omlp.random_params(h, np.random.default_rng(3), scale=1.5)
+    e.mlp_set(h, 1, P)
+    seen = 0
+    for t in range(80):
+        c = rng.uniform(-1, 1, size=(n, 8))
+        for s in (o, e):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+        for k in range(n):
+            f, pr, tg = e.mlp_records(k)
+            assert f.shape[0] == min(int((o.contacts(k)["depth"] > 0).sum()), 16)
+            if f.shape[0]:
+                seen += f.shape[0]
+                assert np.isfinite(f).all() and (tg >= 0).all() and np.isfinite(tg).all()
+                ref, _ = omlp.forward(P, h, f, bf16=True)
+                assert np.abs(pr - ref).max() <= 1e-5 * (1 + np.abs(ref).max()), (t, k)
+    assert seen > 0
+    assert np.array_equal(o.solver_iterations(), e.solver_iterations())
+
+
+def test_learned_warm_start_candidate(ant):
+    """With the candidate on: an all-zero network (lambda = 0, the candidate is
+    a0) changes nothing, bit for bit; a random network gives the same step as
+    the reference rule to the solver tolerance, from the same state."""
+    from oracle import mlp as omlp
+
+    n, h = 8, 32
+    o, e, rng = _ant_pair(ant, n, 22)
+    e.mlp_set(h, 3, np.zeros(omlp.nparam(h), dtype=np.float32))
+    for t in range(60):
+        c = rng.uniform(-1, 1, size=(n, 8))
+        for s in (o, e):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    e.mlp_set(h, 3, omlp.random_params(h, np.random.default_rng(4), scale=1.0))
+    for t in range(40):
+        c = rng.uniform(-1, 1, size=(n, 8))
+        e.set("qpos", 0, n, o.get("qpos"))
+        e.set("qvel", 0, n, o.get("qvel"))
+        for s in (o, e):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.abs(o.get("qvel") - e.get("qvel")).max() < 1e-6, t
