@@ -78,12 +78,13 @@ struct DevLanes {
     uint64_t m = __ballot(pred(lane()) ? 1 : 0);
     return (uint32_t)((m >> (threadIdx.x & 48u)) & 0xFFFFu);
   }
+  // lane J's value in every lane of this lane's 16-lane row: one 64-bit DPP
+  // move (v_mov_b64_dpp row_newbcast:J; every source lane is valid)
   template <int J>
   __device__ __forceinline__ static DLV bc(DLV x) {
-    int lo = __double2loint(x.v), hi = __double2hiint(x.v);
-    lo = __builtin_amdgcn_update_dpp(0, lo, 0x150 + J, 0xF, 0xF, false);  // row_newbcast:J
-    hi = __builtin_amdgcn_update_dpp(0, hi, 0x150 + J, 0xF, 0xF, false);
-    return {__hiloint2double(hi, lo)};
+    long v = __builtin_bit_cast(long, x.v);
+    v = __builtin_amdgcn_mov_dpp(v, 0x150 + J, 0xF, 0xF, true);
+    return {__builtin_bit_cast(double, v)};
   }
 };
 
