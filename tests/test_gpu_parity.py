@@ -153,6 +153,35 @@ def test_specialised_kernel_equals_generic(assets, monkeypatch):
     assert np.array_equal(sims[0].solver_iterations(), sims[1].solver_iterations())
 
 
+@pytest.mark.parametrize("k", ["1", "2"])
+def test_env_packing_matches(assets, monkeypatch, k):
+    """The generic kernel with 1 or 2 envs per wavefront (the packings larger
+    models get) against the oracle, at a ragged tail (n = 37), with the
+    learned-warm-start kernel's logging variant alongside."""
+    arr = mjcf.load_file(assets / "ant.xml")
+    n = 37
+    monkeypatch.setenv("PIPPIN_HIP_ENVS_PER_WG", k)
+    monkeypatch.setenv("PIPPIN_HIP_SPEC", "0")
+    g, lg = HipPhysics(arr, n), HipPhysics(arr, n)
+    lg.mlp_init(hidden=32, seed=1, max_batch=1024)
+    lg.mlp_mode(HipPhysics.MLP_LOG)
+    o = OracleSim(arr, n)
+    rng = np.random.default_rng(8)
+    q = o.get("qpos") + rng.normal(size=(n, 15)) * 0.02
+    q[:, 3:7] /= np.linalg.norm(q[:, 3:7], axis=1, keepdims=True)
+    for s in (o, g, lg):
+        s.set("qpos", 0, n, q)
+    say("env packing", k, "ant", n)
+    for t in range(80):
+        c = rng.uniform(-1, 1, size=(n, 8))
+        for s in (o, g, lg):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+        assert np.array_equal(o.get("qpos"), lg.get("qpos")), t
+    assert np.array_equal(o.solver_iterations(), g.solver_iterations())
+
+
 @pytest.mark.parametrize("name", sorted(MULTIBODY_SCENES))
 def test_multibody_islands_match(assets, name):
     """Several dof blocks: Newton islands formed by contacts (newton.rs:378-473)."""
