@@ -337,7 +337,9 @@ PHD void collide(const MT& M, const DState& S, Env& E) {
            fabs(r[6]) * h.x + fabs(r[7]) * h.y + fabs(r[8]) * h.z));
   });
   int* cnt = reinterpret_cast<int*>(LD(paircnt));
+  int* isc = E.si + M.L.i_iscal;
   auto stage = S.stage + E.e * (int64_t)M.npair * MAXHIT * CW;
+  // broad phase: cnt[p] = 1 for a pair that passes the masks and the AABB test
   X::par(M.npair, [&](int p) {
     int g1 = M.pair_g1[p], g2 = M.pair_g2[p];
     cnt[p] = 0;
@@ -348,6 +350,42 @@ PHD void collide(const MT& M, const DState& S, Env& E) {
     const Real* B = aabb + 6 * g2;
     for (int k = 0; k < 3; k++)
       if (fabs(A[k] - B[k]) > A[3 + k] + B[3 + k] + margin) return;
+    cnt[p] = 1;
+  });
+  // the passing pairs, in pair order, listed in the offset array (free until
+  // the compaction below), so that the narrow phase takes one lane per passing
+  // pair instead of one round per 16 candidate pairs
+  int* list = reinterpret_cast<int*>(LD(pairoff));
+  {
+    using LV = typename X::LV;
+    const int np = M.npair, run = (np + GL - 1) / GL;
+    const LV part = X::gen([&](int l) {
+      int t = 0;
+      for (int p = l * run; p < (l + 1) * run && p < np; p++) t += cnt[p];
+      return (Real)t;
+    });
+    LV pb[GL];
+    bcast_all<X>(part, pb);
+    X::each([&](int l) {
+      int o = 0;
+#pragma unroll
+      for (int k = 0; k < GL; k++)
+        if (k < l) o += (int)X::at(pb[k], l);
+      for (int p = l * run; p < (l + 1) * run && p < np; p++)
+        if (cnt[p]) list[o++] = p;
+      if (l == 0) {
+        int tot = 0;
+#pragma unroll
+        for (int k = 0; k < GL; k++) tot += (int)X::at(pb[k], l);
+        isc[12] = tot;
+      }
+    });
+    X::sync();
+  }
+  const int npass = isc[12];
+  X::par(npass, [&](int i) {
+    const int p = list[i];
+    int g1 = M.pair_g1[p], g2 = M.pair_g2[p];
     GPose a, b;
     a.typ = M.geom_type[g1];
     a.pos = ld3(LD(gxpos) + 3 * g1);
@@ -367,7 +405,6 @@ PHD void collide(const MT& M, const DState& S, Env& E) {
   // order): each lane of the 16-lane row sums a contiguous run of pairs, the
   // run totals are scanned across lanes, then each lane writes its offsets
   int* off = reinterpret_cast<int*>(LD(pairoff));
-  int* isc = E.si + M.L.i_iscal;
   {
     using LV = typename X::LV;
     const int np = M.npair, run = (np + GL - 1) / GL;
