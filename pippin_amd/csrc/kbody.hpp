@@ -173,19 +173,39 @@ PHD void geom_poses(const MT& M, Env& E) {
   });
 }
 
-// x[parent(b)] += x[b] for b from the last body down to 1 (w components per
-// body), level by level: lanes take (parent, component) pairs of one level.
-template <class X, class MT>
-PHD void leaf_to_root(const MT& M, Real* x, int w) {
+// x[parent(b)] += x[b] for b from the last body down to 1 (W components per
+// body), level by level. A level with at most 16 (parent, component) pairs
+// gives each pair a lane; a larger one gives each parent a lane that sums all
+// W components side by side (independent chains, one pass over the schedule).
+// Each component adds the children in the same order either way.
+template <class X, int W, class MT>
+PHD void leaf_to_root(const MT& M, Real* x) {
   for (int L = 0; L < M.nacc; L++) {
     int g0 = M.acc_level_adr[L], ng = M.acc_level_adr[L + 1] - g0;
-    X::par(ng * w, [&](int t) {
-      int g = g0 + t / w, c = t % w;
-      int p = M.acc_parent[g];
-      Real v = x[w * p + c];
-      for (int k = M.acc_child_adr[g]; k < M.acc_child_adr[g + 1]; k++) v = v + x[w * M.acc_child[k] + c];
-      x[w * p + c] = v;
-    });
+    if (ng * W <= GL) {
+      X::par(ng * W, [&](int t) {
+        int g = g0 + t / W, c = t % W;
+        int p = M.acc_parent[g];
+        Real v = x[W * p + c];
+        for (int k = M.acc_child_adr[g]; k < M.acc_child_adr[g + 1]; k++) v = v + x[W * M.acc_child[k] + c];
+        x[W * p + c] = v;
+      });
+    } else {
+      X::par(ng, [&](int i) {
+        const int g = g0 + i;
+        const int p = M.acc_parent[g];
+        Real v[W];
+#pragma unroll
+        for (int c = 0; c < W; c++) v[c] = x[W * p + c];
+        for (int k = M.acc_child_adr[g]; k < M.acc_child_adr[g + 1]; k++) {
+          const Real* xc = x + W * M.acc_child[k];
+#pragma unroll
+          for (int c = 0; c < W; c++) v[c] = v[c] + xc[c];
+        }
+#pragma unroll
+        for (int c = 0; c < W; c++) x[W * p + c] = v[c];
+      });
+    }
   }
 }
 
@@ -199,17 +219,23 @@ PHD void crba(const MT& M, Env& E) {
   // composite inertia leaf to root: each of the 13 components is its own
   // sequence of additions in the reference's (decreasing body) order
   X::par(13 * nb, [&](int t) { crb[t] = ci[t]; });
-  leaf_to_root<X>(M, crb, 13);
+  leaf_to_root<X, 13>(M, crb);
   X::par(nv * nv, [&](int k) { qm[k] = 0.0; });
   X::par(nv, [&](int i) {
     const Real* cd = LD(cdof);
     SP f = mul_motion(crb + 13 * M.dof_body[i], ldsp(cd + 6 * i));
+    // i, then the dof_parent chain; that chain is the set of strict
+    // ancestors in decreasing index order (parents precede children), read
+    // from the ancestor mask instead of one dependent load per step
+    uint64_t anc = M.dof_anc[i];
     int j = i;
-    while (j >= 0) {
+    while (true) {
       Real v = sdot(ldsp(cd + 6 * j), f);
       qm[i * nv + j] = v;
       qm[j * nv + i] = v;
-      j = M.dof_parent[j];
+      if (!anc) break;
+      j = 63 - __builtin_clzll(anc);
+      anc &= ~((uint64_t)1 << j);
     }
     qm[i * nv + i] += M.dof_armature[i];
   });
@@ -264,7 +290,7 @@ PHD void velocity_rne(const MT& M, Env& E) {
       stsp(cfrc + 6 * b, spadd(mul_motion(ci + 13 * b, a), cross_force(v, h)));
     });
   }
-  leaf_to_root<X>(M, cfrc, 6);
+  leaf_to_root<X, 6>(M, cfrc);
   X::par(M.nv, [&](int k) {
     LD(bias)[k] = sdot(ldsp(LD(cdof) + 6 * k), ldsp(cfrc + 6 * M.dof_body[k]));
   });
