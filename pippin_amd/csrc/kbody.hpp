@@ -887,8 +887,6 @@ PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
     Real v = rdd[clampi(l, nr)];
     return l < nr ? v : 0.0;
   });
-  LV ddb[GL];
-  bcast_all<X>(dd, ddb);
   // active(v): bilateral rows always, inequality rows when v < 0
   auto active_mask = [&](const LV& v) {
     return X::ballot([&](int l) { return l < nr && (l >= n_ineq || X::at(v, l) < 0.0); });
@@ -1031,20 +1029,26 @@ PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
     const Real mpp = X::uni(lsum_lanes<X>(p * mp, nv, -0.0));
     const Real mpg = X::uni(lsum_lanes<X>(tmp * mp, nv, -0.0));
     const LV jp = jdot(p);
-    LV jarb[GL], jpb[GL];
-    bcast_all<X>(jar, jarb);
-    bcast_all<X>(jp, jpb);
+    // Row r's terms, one row per lane: D_r jp_r v_r and D_r jp_r^2 when the
+    // row is active at step al, v_r = jar_r + al jp_r; they are then added in
+    // row order. An inactive row (or a lane past nr) adds +0.0, which leaves
+    // the sums equal except for the sign of an exact zero, and no later use
+    // (|d1|, d1 < 0, al - d1 / max(d2, 1e-30)) depends on that sign.
+    const LV dq = dd * jp;
+    const LV dq2 = dq * jp;
     auto deriv = [&](Real al, Real* d1o, Real* d2o) {
+      const LV v = jar + al * jp;
+      auto on = [&](int l) { return l < nr && (l >= n_ineq || X::at(v, l) < 0.0); };
+      const LV t1 = X::sel(on, dq * v, X::splat(0.0));
+      const LV t2 = X::sel(on, dq2, X::splat(0.0));
+      LV b1[GL], b2[GL];
+      bcast_all<X>(t1, b1);
+      bcast_all<X>(t2, b2);
       Real d1 = al * mpp + mpg, d2 = mpp;
 #pragma unroll
       for (int r = 0; r < GL; r++) {
-        Real jpr = X::uni(jpb[r]);
-        Real v = X::uni(jarb[r]) + al * jpr;
-        Real dq = X::uni(ddb[r]) * jpr;
-        bool on = r < nr && (r >= n_ineq || v < 0.0);
-        Real e1 = d1 + dq * v, e2 = d2 + dq * jpr;
-        d1 = on ? e1 : d1;
-        d2 = on ? e2 : d2;
+        d1 = d1 + X::uni(b1[r]);
+        d2 = d2 + X::uni(b2[r]);
       }
       *d1o = d1;
       *d2o = d2;
