@@ -27,6 +27,51 @@ PHD Real rmin(Real a, Real b) { return fmin(a, b); }
 PHD bool rfinite(Real x) { return x - x == 0.0; }
 PHD Real rsignum(Real x) { return x != x ? x : copysign(1.0, x); }
 
+// IEEE (correctly rounded) square root. On the device, for 2^-767 <= x <=
+// DBL_MAX this is the compiler's own f64 sqrt sequence (rsq estimate, one
+// Goldschmidt refinement of the root and half-reciprocal, two residual
+// corrections) without the input scaling and special-value selects that the
+// full sequence applies outside that range, where both give the same bits;
+// other inputs take the full sequence.
+PHD Real sqrt_cr(Real x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if (x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023) {
+    const Real y = __builtin_amdgcn_rsq(x);
+    Real g = x * y, h = y * 0.5;
+    const Real r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    Real d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+  }
+  return __builtin_sqrt(x);
+#else
+  return sqrt(x);
+#endif
+}
+
+// 1.0 / s, correctly rounded, for s a square root of a positive finite double
+// (so 2^-537 <= s <= 2^512). On the device this is the compiler's f64 division
+// sequence with numerator 1 (reciprocal estimate, two refinements, quotient,
+// residual correction); for such s its operand scaling and special-value fixup
+// leave every value unchanged, so they are left out. A NaN s gives NaN.
+PHD Real recip_of_root(Real s) {
+#ifdef __HIP_DEVICE_COMPILE__
+  Real r = __builtin_amdgcn_rcp(s);
+  Real e = fma(-s, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-s, r, 1.0);
+  r = fma(r, e, r);
+  const Real q = 1.0 * r;
+  e = fma(-s, q, 1.0);
+  return fma(e, r, q);
+#else
+  return 1.0 / s;
+#endif
+}
+
 // ---- correctly rounded sin and cos ----
 // The reference calls the platform libm (Rust f64::sin_cos); libms differ from
 // each other by up to one ulp, and through contact one ulp grows into visible
@@ -166,7 +211,7 @@ PHD V3 operator*(V3 a, Real s) { return v3(a.x * s, a.y * s, a.z * s); }
 PHD Real dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 PHD V3 cross(V3 a, V3 o) { return v3(a.y * o.z - a.z * o.y, a.z * o.x - a.x * o.z, a.x * o.y - a.y * o.x); }
 PHD Real norm2(V3 a) { return dot(a, a); }
-PHD Real norm(V3 a) { return sqrt(norm2(a)); }
+PHD Real norm(V3 a) { return sqrt_cr(norm2(a)); }
 PHD V3 normalized(V3 a) {
   Real n = norm(a);
   if (n > 1e-15) return a * (1.0 / n);
@@ -255,7 +300,7 @@ PHD Q4 qnormalized(Q4 q) {
   s += q.x * q.x;
   s += q.y * q.y;
   s += q.z * q.z;
-  Real n = sqrt(s);
+  Real n = sqrt_cr(s);
   if (n < 1e-15) return q4(1, 0, 0, 0);
   return q4(q.w / n, q.x / n, q.y / n, q.z / n);
 }

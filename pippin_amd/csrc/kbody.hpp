@@ -977,7 +977,7 @@ PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
     }
   }
   const LV ma0 = mul_m(a0);
-  const Real scale = 1.0 + sqrt(X::uni(lsum_lanes<X>(ma0 * ma0, nv, -0.0)));
+  const Real scale = 1.0 + sqrt_cr(X::uni(lsum_lanes<X>(ma0 * ma0, nv, -0.0)));
   const Real tol = M.tolerance;
   const int cap = M.iterations > 1 ? M.iterations : 1;
   uint64_t prev = ~(uint64_t)0;
@@ -995,7 +995,7 @@ PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
     act_a = act;
     fresh = true;
     const LV grad = add_rows(mul_m(tmp), dd * jar, act);
-    const Real gnorm = sqrt(X::uni(lsum_lanes<X>(grad * grad, nv, -0.0)));
+    const Real gnorm = sqrt_cr(X::uni(lsum_lanes<X>(grad * grad, nv, -0.0)));
     if (gnorm < tol * scale || ((uint64_t)act == prev && iters > 1)) break;
     prev = act;
     // H = M + sum_active D_r J_r^T J_r, row l in lane l

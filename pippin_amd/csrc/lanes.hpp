@@ -69,7 +69,8 @@ struct DevLanes {
   __device__ __forceinline__ static DLV sel(P&& pred, DLV a, DLV b) {
     return pred(lane()) ? a : b;
   }
-  __device__ __forceinline__ static DLV sqrt_(DLV a) { return {sqrt(a.v)}; }
+  __device__ __forceinline__ static DLV sqrt_(DLV a) { return {sqrt_cr(a.v)}; }
+  __device__ __forceinline__ static DLV recip_root_(DLV a) { return {recip_of_root(a.v)}; }
   // the value of a group-uniform vector
   __device__ __forceinline__ static Real uni(DLV a) { return a.v; }
   // bit l set when pred holds on lane l of this lane's 16-lane row
@@ -156,6 +157,11 @@ struct HostLanes {
   static HLV sqrt_(const HLV& a) {
     HLV r;
     for (int l = 0; l < GL; l++) r.v[l] = sqrt(a.v[l]);
+    return r;
+  }
+  static HLV recip_root_(HLV a) {
+    HLV r;
+    for (int l = 0; l < GL; l++) r.v[l] = 1.0 / a.v[l];
     return r;
   }
   static Real uni(const HLV& a) { return a.v[0]; }
@@ -272,7 +278,7 @@ PHD bool chol_rows(LV* r, int n) {
       return false;
     }
     LV ljj = X::sqrt_(dj);
-    LV inv = 1.0 / ljj;
+    LV inv = X::recip_root_(ljj);  // 1.0 / ljj
     r[j] = X::sel([](int l) { return l == j; }, ljj, X::sel([](int l) { return l > j; }, d * inv, r[j]));
     return true;
   });

@@ -106,8 +106,10 @@ __global__ void __launch_bounds__(64) k_selftest_math(const Real* x, int64_t n, 
     cr_sincos(x[i], &s, &c);
     out[4 * i] = s;
     out[4 * i + 1] = c;
-    out[4 * i + 2] = sqrt(fabs(x[i]));
-    out[4 * i + 3] = 1.0 / x[i];
+    out[4 * i + 2] = sqrt_cr(fabs(x[i]));  // the kernels' square root
+    // the kernels' reciprocal of a root over its domain, division elsewhere
+    const Real a = x[i];
+    out[4 * i + 3] = (a >= 0x1p-537 && a <= 0x1p512) ? recip_of_root(a) : 1.0 / a;
   }
 }
 

@@ -27,15 +27,25 @@ def say(*a):
 
 def test_device_math_matches_host():
     rng = np.random.default_rng(7)
+    # square roots and reciprocals over the whole range, including both sides
+    # of the shortcut bounds (2^-767 for sqrt, [2^-537, 2^512] for 1/x)
+    wide = np.exp2(rng.uniform(-1070, 1023, 200000))
+    edges = np.array([2.0 ** -767, 2.0 ** -537, 2.0 ** 512, 2.0 ** 1023, 5e-324, 2.2250738585072014e-308,
+                      1.7976931348623157e308])
+    with np.errstate(over="ignore"):
+        edges = np.concatenate([edges, np.nextafter(edges, 0), np.nextafter(edges, np.inf)])
     x = np.concatenate([rng.uniform(-20, 20, 20000), rng.uniform(-1e-3, 1e-3, 2000),
-                        [0.0, -0.0, 1e-300, math.pi / 4, math.pi / 2, 1.0, -2.5, 3.0]])
+                        [0.0, -0.0, 1e-300, math.pi / 4, math.pi / 2, 1.0, -2.5, 3.0],
+                        wide, -wide[:1000], edges, np.array([np.inf, -np.inf, np.nan])])
+    fin = np.isfinite(x)
     say("device math on", x.size, "inputs")
     out = selftest_math(x)
-    s, c = cr_sincos(x)
-    assert np.array_equal(out[:, 0], s) and np.array_equal(out[:, 1], c), "cr_sincos differs from the oracle's"
-    assert np.array_equal(out[:, 2], np.sqrt(np.abs(x))), "device sqrt is not correctly rounded"
-    with np.errstate(divide="ignore"):
-        assert np.array_equal(out[:, 3], 1.0 / x), "device division is not correctly rounded"
+    small = fin & (np.abs(x) < 1e6)  # cr_sincos is checked on the range the kernels use
+    s, c = cr_sincos(x[small])
+    assert np.array_equal(out[small, 0], s) and np.array_equal(out[small, 1], c), "cr_sincos differs"
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        assert np.array_equal(out[:, 2], np.sqrt(np.abs(x)), equal_nan=True), "device sqrt is not correctly rounded"
+        assert np.array_equal(out[:, 3], 1.0 / x, equal_nan=True), "device reciprocal is not correctly rounded"
 
 
 def _ant(assets, n):
