@@ -262,3 +262,28 @@ def test_learned_warm_start_candidate(ant):
             s.set("ctrl", 0, n, c)
             s.step(0, n, 1)
         assert np.abs(o.get("qvel") - e.get("qvel")).max() < 1e-6, t
+
+
+@pytest.mark.parametrize("name", ["arm3d.xml", "ant.xml"])
+def test_implicitfast_exact(assets, name):
+    """integrator="implicitfast" (forward.rs:407-431): the position and velocity
+    actuators' kv terms on tree-ancestor dof pairs (arm3d) and joint damping
+    (both), through the general factorisation path."""
+    xml = (assets / name).read_text()
+    xml = xml.replace("<option ", '<option integrator="implicitfast" ', 1)
+    arr = mjcf.load_string(xml)
+    assert int(arr["integrator"]) == 1
+    n = 6
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(17)
+    v = rng.normal(size=(n, int(arr["nv"]))) * 0.5
+    for s in (o, e):
+        s.set("qvel", 0, n, v)
+    nu = int(arr["nu"])
+    for t in range(150):
+        c = rng.uniform(-1, 1, size=(n, nu))
+        for s in (o, e):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert np.array_equal(o.get("qvel"), e.get("qvel"))

@@ -261,3 +261,30 @@ def test_pgs_multibody_match(assets, name):
             s.step(0, n, 1)
         assert np.array_equal(o.ncon(), g.ncon()), t
     assert np.array_equal(o.get("qpos"), g.get("qpos"))
+
+
+@pytest.mark.parametrize("name", ["arm3d.xml", "ant.xml"])
+def test_implicitfast_match(assets, name):
+    """integrator="implicitfast": actuator kv terms on tree-ancestor dof pairs
+    and joint damping in the factorisation (forward.rs:392-456)."""
+    xml = (assets / name).read_text().replace("<option ", '<option integrator="implicitfast" ', 1)
+    arr = mjcf.load_string(xml)
+    n = 48
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(17)
+    v = rng.normal(size=(n, int(arr["nv"]))) * 0.5
+    for s in (o, g):
+        s.set("qvel", 0, n, v)
+    nu = int(arr["nu"])
+    say("implicitfast", name, n)
+    contacts = 0
+    for t in range(150):
+        c = rng.uniform(-1, 1, size=(n, nu))
+        for s in (o, g):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        contacts += int(o.ncon().sum())
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert np.array_equal(o.get("qvel"), g.get("qvel"))
+    say("  contacts", contacts)
