@@ -271,13 +271,21 @@ PHD void capsule_box(const GPose& a, const GPose& b, Real margin, Sink& out) {
 }
 
 // collision.rs:496-511: keep the part of poly with pn.p <= off.
+// Capacity of the clipping polygon. One clip keeps the vertices on the inner
+// side (distance <= 0) and adds one point per sign change; for a convex
+// polygon there are at most two, so a clip adds at most two vertices and the
+// face quad has at most 12 after its four clips (vertices on a clip line
+// really do produce them: 9 to 12 occur when two box faces are parallel).
+// The reference's vectors are unbounded; here a vertex past the capacity
+// would be dropped rather than written out of bounds.
+constexpr int CLIP_CAP = 24;
 PHD int clip(const V3* poly, int n, V3 pn, Real off, V3* out) {
   int m = 0;
   for (int i = 0; i < n; i++) {
     V3 a = poly[i], b = poly[(i + 1) % n];
     Real da = dot(pn, a) - off, db = dot(pn, b) - off;
-    if (da <= 0.0) out[m++] = a;
-    if ((da < 0.0) != (db < 0.0) && fabs(da - db) > 1e-15) out[m++] = a + (b - a) * (da / (da - db));
+    if (da <= 0.0 && m < CLIP_CAP) out[m++] = a;
+    if ((da < 0.0) != (db < 0.0) && fabs(da - db) > 1e-15 && m < CLIP_CAP) out[m++] = a + (b - a) * (da / (da - db));
   }
   return m;
 }
@@ -364,7 +372,7 @@ PHD void box_box(const GPose& a, const GPose& b, Real margin, Sink& out) {
   int ku = (j + 1) % 3, kv = (j + 2) % 3;
   V3 eu = iaxes[ku] * inc.size[ku];
   V3 ev = iaxes[kv] * inc.size[kv];
-  V3 poly[8], tmp[8];
+  V3 poly[CLIP_CAP], tmp[CLIP_CAP];
   poly[0] = ic + eu + ev;
   poly[1] = ic - eu + ev;
   poly[2] = ic - eu - ev;
@@ -381,8 +389,8 @@ PHD void box_box(const GPose& a, const GPose& b, Real margin, Sink& out) {
       if (np == 0) return;
     }
   }
-  Real sep[8];
-  V3 pts[8];
+  Real sep[CLIP_CAP];
+  V3 pts[CLIP_CAP];
   int npt = 0;
   for (int q = 0; q < np; q++) {
     Real sp_ = dot(poly[q] - ref_center, nref);

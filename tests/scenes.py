@@ -34,3 +34,63 @@ MULTIBODY_SCENES = {
             '<body pos="-.4 0 .2"><freejoint/><geom type="box" size=".04 .03 .05"/></body>'
             '</worldbody></mujoco>',
 }
+
+# Arm with a two-finger gripper and a free cube on the floor, after the
+# reference's arm + gripper configuration: a fixed tendon drives both fingers
+# (position actuator), a joint equality couples the fingers, the integrator is
+# implicitfast, and the arm and the cube are separate dof blocks joined into
+# one Newton island only while they touch.
+GRIPPER = """<mujoco model="gripper">
+  <option timestep="0.004" integrator="implicitfast"/>
+  <default>
+    <joint damping="0.5" armature="0.01"/>
+    <geom friction="0.8"/>
+  </default>
+  <worldbody>
+    <geom type="plane" size="2 2 .1"/>
+    <body name="base" pos="0 0 0.45">
+      <joint name="yaw" axis="0 0 1"/>
+      <geom type="capsule" fromto="0 0 0 0 0 0.05" size="0.04" contype="0" conaffinity="0"/>
+      <body name="upper" pos="0 0 0.05">
+        <joint name="shoulder" axis="0 1 0" range="-1.5 1.5" limited="true"/>
+        <geom type="capsule" fromto="0 0 0 0.25 0 0" size="0.03" contype="0" conaffinity="0"/>
+        <body name="fore" pos="0.25 0 0">
+          <joint name="elbow" axis="0 1 0" range="-2 2" limited="true"/>
+          <geom type="capsule" fromto="0 0 0 0 0 -0.25" size="0.025" contype="0" conaffinity="0"/>
+          <body name="palm" pos="0 0 -0.27">
+            <geom type="box" size="0.04 0.05 0.01"/>
+            <body name="f1" pos="0 0.03 -0.04">
+              <joint name="finger1" type="slide" axis="0 1 0" range="-0.03 0.03" limited="true"/>
+              <geom type="box" size="0.01 0.005 0.03"/>
+            </body>
+            <body name="f2" pos="0 -0.03 -0.04">
+              <joint name="finger2" type="slide" axis="0 -1 0" range="-0.03 0.03" limited="true"/>
+              <geom type="box" size="0.01 0.005 0.03"/>
+            </body>
+          </body>
+        </body>
+      </body>
+    </body>
+    <body name="cube" pos="0.25 0 0.03">
+      <freejoint/>
+      <geom type="box" size="0.025 0.025 0.025"/>
+    </body>
+  </worldbody>
+  <tendon>
+    <fixed name="grip">
+      <joint joint="finger1" coef="0.5"/>
+      <joint joint="finger2" coef="0.5"/>
+    </fixed>
+  </tendon>
+  <equality>
+    <joint joint1="finger2" joint2="finger1" polycoef="0 1 0 0 0"/>
+  </equality>
+  <actuator>
+    <position joint="yaw" kp="20" kv="2"/>
+    <position joint="shoulder" kp="40" kv="4"/>
+    <position joint="elbow" kp="40" kv="4"/>
+    <position tendon="grip" kp="50" kv="1" ctrlrange="-0.03 0.03" ctrllimited="true"/>
+  </actuator>
+</mujoco>
+"""
+

@@ -287,3 +287,29 @@ def test_implicitfast_exact(assets, name):
             s.step(0, n, 1)
         assert np.array_equal(o.get("qpos"), e.get("qpos")), t
     assert np.array_equal(o.get("qvel"), e.get("qvel"))
+
+
+def test_gripper_exact():
+    """Arm + gripper + cube (tests/scenes.py GRIPPER): fixed-tendon position
+    actuator, joint equality between the fingers, implicitfast, joint limits,
+    and the arm and cube dof blocks joined into one island by contact."""
+    from tests.scenes import GRIPPER
+
+    arr = mjcf.load_string(GRIPPER)
+    n = 4
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(31)
+    q = o.get("qpos")
+    q[:, 5:7] += rng.normal(size=(n, 2)) * 0.01
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+    for t in range(300):
+        c = np.stack([np.sin(t * 0.01 + np.arange(n)), 0.4 + 0.3 * np.sin(t * 0.013) + 0 * np.arange(n),
+                      -0.9 + 0.5 * np.sin(t * 0.017) + 0 * np.arange(n), 0.03 * np.sin(t * 0.02 + np.arange(n))], 1)
+        for s in (o, e):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert np.array_equal(o.solver_iterations(), e.solver_iterations())
+    assert o.ncon().sum() > 0

@@ -288,3 +288,29 @@ def test_implicitfast_match(assets, name):
         assert np.array_equal(o.get("qpos"), g.get("qpos")), t
     assert np.array_equal(o.get("qvel"), g.get("qvel"))
     say("  contacts", contacts)
+
+
+def test_gripper_match():
+    """Arm + gripper + cube: fixed tendon, joint equality, implicitfast, limits,
+    islands joined by contact."""
+    from tests.scenes import GRIPPER
+
+    arr = mjcf.load_string(GRIPPER)
+    n = 32
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(31)
+    q = o.get("qpos")
+    q[:, 5:7] += rng.normal(size=(n, 2)) * 0.01
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    say("gripper", n)
+    k = np.arange(n)
+    for t in range(300):
+        c = np.stack([np.sin(t * 0.01 + k), 0.4 + 0.3 * np.sin(t * 0.013 + 0 * k),
+                      -0.9 + 0.5 * np.sin(t * 0.017 + 0 * k), 0.03 * np.sin(t * 0.02 + k)], 1)
+        for s in (o, g):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert np.array_equal(o.solver_iterations(), g.solver_iterations())
