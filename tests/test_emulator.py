@@ -313,3 +313,21 @@ def test_gripper_exact():
         assert np.array_equal(o.get("qpos"), e.get("qpos")), t
     assert np.array_equal(o.solver_iterations(), e.solver_iterations())
     assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("seed", [3, 11, 29])
+@pytest.mark.parametrize("solver", [0, 1])
+def test_random_piles_exact(seed, solver):
+    """Random primitive piles (tests/scenes.py random_pile_xml), Newton and PGS."""
+    from tests.scenes import random_pile_xml
+
+    arr = mjcf.load_string(random_pile_xml(seed))
+    arr["solver_kind"] = solver
+    n = 3
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    for t in range(200):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert o.ncon().sum() > 0

@@ -314,3 +314,28 @@ def test_gripper_match():
         assert np.array_equal(o.ncon(), g.ncon()), t
         assert np.array_equal(o.get("qpos"), g.get("qpos")), t
     assert np.array_equal(o.solver_iterations(), g.solver_iterations())
+
+
+@pytest.mark.parametrize("seed", [3, 11, 29, 41])
+@pytest.mark.parametrize("solver", ["newton", "pgs"])
+def test_random_piles_match(seed, solver):
+    """Random primitive piles with per-env perturbed drops, Newton and PGS."""
+    from tests.scenes import random_pile_xml
+
+    arr = mjcf.load_string(random_pile_xml(seed))
+    arr["solver_kind"] = 0 if solver == "newton" else 1
+    n = 24
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(seed)
+    q = o.get("qpos")
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    say("random pile", seed, solver, n)
+    for t in range(250):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert o.ncon().sum() > 0
