@@ -170,6 +170,53 @@ def warm_start_study(model: str, n: int, sub: int, record: int, train_iters: int
             "kernel_ms_per_step_reference": kr / lr_, "kernel_ms_per_step_learned": km / lm}
 
 
+def pile_study(n: int, steps: int, seed: int, cpu_envs: int):
+    """Tabletop piles (pippin_amd/workloads.py random_pile_xml: 2 to 6 free
+    primitives dropped onto a plane), after the reference's tumble scenes:
+    env-steps/s on the GPU (4 physics steps per launch, bodies settling and
+    resting) and of the oracle on the host cores over a sample of the same
+    envs and steps, for both solvers."""
+    import torch
+    from oracle.pyoracle import OracleSim
+    from pippin_amd.physics import HipPhysics, load_model
+    from pippin_amd.workloads import random_pile_xml
+
+    threads = len(os.sched_getaffinity(0))
+    arrays = load_model(random_pile_xml(seed), is_xml=True)
+    out = []
+    for solver in ("newton", "pgs"):
+        sim = HipPhysics(arrays, n, solver=solver)
+        rng = np.random.default_rng(seed)
+        q = sim.get("qpos")
+        nb = int(arrays["nq"]) // 7
+        for b in range(nb):
+            q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+        sim.set("qpos", 0, n, q)
+        sim.step(0, n, 40)  # the drop
+        torch.cuda.synchronize()
+        sim.kernel_timing(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(steps // 4):
+            sim.step(0, n, 4)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        kms, launches = sim.kernel_timing(reset=True)
+        m = min(cpu_envs, n)
+        ref = OracleSim(arrays, m)
+        ref.set("qpos", 0, m, q[:m])
+        ref.step(0, m, 40, threads=threads)
+        c0 = time.perf_counter()
+        ref.step(0, m, steps, threads=threads)
+        cdt = time.perf_counter() - c0
+        out.append({"workload": f"pile seed {seed} ({nb} bodies), {solver}", "envs": n, "steps": steps,
+                    "gpu_env_steps_per_s": n * steps / dt, "kernel_ms_per_launch": kms / launches,
+                    "cpu_env_steps_per_s": m * steps / cdt, "cpu_envs": m, "cpu_threads": threads,
+                    "newton_iters_mean": float(sim.solver_iterations().mean()),
+                    "contacts_mean": float(sim.ncon().mean())})
+        sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,7 +236,15 @@ def main():
     ap.add_argument("--warm-start", action="store_true",
                     help="only run the learned warm-start study (Ant and box_drop: mean Newton iterations "
                          "with and without the candidate) and print one JSON line per model")
+    ap.add_argument("--piles", action="store_true",
+                    help="only run the tabletop pile study (8192 envs, both solvers, GPU and host oracle) "
+                         "and print one JSON line per solver")
     args = ap.parse_args()
+    if args.piles:
+        for seed in (11, 29):
+            for line in pile_study(8192, 200, seed, 1024):
+                print(json.dumps(line), flush=True)
+        return
     if args.warm_start:
         for model, n, sub, rec, meas in (("ant.xml", args.envs, args.substeps, 100, 25),
                                           ("box_drop.xml", args.envs, 1, 400, 100)):

@@ -1546,16 +1546,23 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
   const Real a3 = 1.0 / (1.0 + a2);
   const Real bias_rate = omega / a1, mass_scale = a2 * a3, impulse_scale = a3;
   const Real max_bias = 1.0, pgs_tol = 1e-5;
-  X::par(1, [&](int) {
-    auto iterate = [&](bool use_bias) {
-      Real max_dv = 0.0;
-      for (int r = 0; r < nr; r++) {
-        Real* q = rw + r * 10;
-        const Real inv_a = q[PR_INVA];
-        if (inv_a == 0.0) continue;
-        const Real* jr = J + r * nv;
+  // Gauss-Seidel sweeps (solver.rs:149-245): rows in order; for each row the
+  // env's lanes form the products J_rk v_k and, after one lane has summed
+  // them in dof order and updated the row's force, the update v += M^-1 J_r^T
+  // dlambda, so that the rows' HBM data is read by all lanes at once
+  Real* prod = LD(tmp);
+  Real* sc = LD(scal);
+  auto iterate = [&](bool use_bias) {
+    X::par(1, [&](int) { sc[8] = 0.0; });  // largest velocity change of the sweep
+    for (int r = 0; r < nr; r++) {
+      Real* q = rw + r * 10;
+      const Real inv_a = q[PR_INVA];
+      if (inv_a == 0.0) continue;
+      const Real* jr = J + r * nv;
+      X::par(nv, [&](int k) { prod[k] = jr[k] * v[k]; });
+      X::par(1, [&](int) {
         Real jv = rsum0();
-        for (int k = 0; k < nv; k++) jv += jr[k] * v[k];
+        for (int k = 0; k < nv; k++) jv += prod[k];
         const Real lam = q[PR_LAM];
         const int kind = (int)q[PR_KIND];
         Real nw;
@@ -1593,23 +1600,27 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
           Real bound = q[PR_MU] * rw[(int)q[PR_NORMAL] * 10 + PR_LAM];
           nw = rclamp(lam - inv_a * jv, -bound, bound);
         }
-        Real dl = nw - lam;
+        const Real dl = nw - lam;
+        sc[9] = dl;
         if (dl != 0.0) {
-          max_dv = rmax(max_dv, fabs(dl) / inv_a);
-          const Real* mr = mj + r * nv;
-          for (int k = 0; k < nv; k++) v[k] += mr[k] * dl;
+          sc[8] = rmax(sc[8], fabs(dl) / inv_a);
           q[PR_LAM] = nw;
         }
+      });
+      const Real dl = sc[9];
+      if (dl != 0.0) {
+        const Real* mr = mj + r * nv;
+        X::par(nv, [&](int k) { v[k] += mr[k] * dl; });
       }
-      return max_dv;
-    };
-    for (int it = 0; it < M.iterations; it++)
-      if (iterate(true) < pgs_tol) break;
-    for (int k = 0; k < nv; k++) vpos[k] = v[k];
-    int relax = M.iterations / 3 > 2 ? M.iterations / 3 : 2;
-    for (int it = 0; it < relax; it++)
-      if (iterate(false) < pgs_tol) break;
-  });
+    }
+    return sc[8];
+  };
+  for (int it = 0; it < M.iterations; it++)
+    if (iterate(true) < pgs_tol) break;
+  X::par(nv, [&](int k) { vpos[k] = v[k]; });
+  const int relax = M.iterations / 3 > 2 ? M.iterations / 3 : 2;
+  for (int it = 0; it < relax; it++)
+    if (iterate(false) < pgs_tol) break;
   // qfrc_constraint = sum_r J_r lambda_r / h, rows in order
   X::par(nv, [&](int k) {
     Real f = 0.0;
