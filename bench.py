@@ -65,8 +65,10 @@ def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
     thread pool. Also counts the FP64 operations of the reference algorithm on
     that window (op-counting build, 256 envs), the roofline's numerator.
     """
+    from oracle import pyoracle
     from oracle.pyoracle import OracleSim, ops_per_env_step
 
+    tuned = pyoracle.use_host_tuned_build()
     threads = len(os.sched_getaffinity(0))
     nact, n, nu = ctrl.shape
     sample = min(n, sample)
@@ -89,7 +91,8 @@ def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
         "kind": "port",
         "sample": f"Ant, envs 0..{sample} of {n}, the {nact - warmup} timed actions x {substeps} substeps "
                   f"after the same {warmup} warmup actions ({env_steps} env-steps in {dt:.1f} s), "
-                  f"Physics::step over the range, chunk = n/(4 x {threads} threads)",
+                  f"Physics::step over the range, chunk = n/(4 x {threads} threads), "
+                  f"oracle built {'-march=x86-64-v3' if tuned else 'for plain x86-64'} -O3 -ffp-contract=off",
     }, ops
 
 

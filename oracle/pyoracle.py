@@ -31,6 +31,23 @@ def build():
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
+def use_host_tuned_build() -> bool:
+    """Load the x86-64-v3 build (same arithmetic, vectorised) when the host CPU
+    has AVX2 and nothing has loaded the library yet; returns whether it did."""
+    global LIB_PATH
+    if _lib is not None:
+        return LIB_PATH.name == "liborc_v3.so"
+    try:
+        flags = open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+    v3 = HERE / "_build" / "liborc_v3.so"
+    if " avx2" in flags and " bmi2" in flags and " fma" in flags and v3.exists():
+        LIB_PATH = v3
+        return True
+    return False
+
+
 def lib():
     global _lib
     if _lib is None:
