@@ -215,3 +215,36 @@ def test_masks_cannot_enable_general_pairs(assets):
         g.set_param("geom_conaffinity", 2, 1, 2, [1.0])  # cylinder (geom 0) x box (geom 2)
     assert np.array_equal(g.get_param("geom_conaffinity", 2)[:, 0], [0, 0, 0])
     assert np.array_equal(g.get_param("geom_contype", 0)[:, 0], [1, 1, 1])
+
+
+def test_contact_capacity_overflow_fails_loudly_and_freezes_the_env(assets):
+    """A user-lowered contact capacity (options.max_contacts) that a step
+    exceeds: the step reports the failure (the reference would panic), the
+    failed env keeps its last state, the others go on stepping."""
+    from pippin_amd.physics import HipError
+
+    arr = mjcf.load_file(assets / "box_drop.xml")
+    n = 3
+    g = HipPhysics(arr, n, max_contacts=2)  # a box resting on the plane makes 4
+    ok = HipPhysics(arr, n)
+    q = g.get("qpos")
+    q[1, [2, 9, 16]] += 5.0  # env 1's three bodies start high up: no contact in time
+    for s in (g, ok):
+        s.set("qpos", 0, n, q)
+    failed_at = None
+    for t in range(200):
+        try:
+            g.step(0, n, 1)
+        except HipError as e:
+            assert "capacity" in str(e)
+            failed_at = t
+            break
+        ok.step(0, n, 1)
+    assert failed_at is not None
+    frozen = g.get("qpos")
+    g.step(1, 2, 5)  # the env that has not failed still steps
+    after = g.get("qpos")
+    assert not np.array_equal(after[1], frozen[1])
+    with pytest.raises(HipError):
+        g.step(0, 1, 1)  # the failed env reports again and stays put
+    assert np.array_equal(g.get("qpos")[0], frozen[0])
