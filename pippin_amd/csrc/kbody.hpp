@@ -1345,6 +1345,29 @@ PHD int newton_solve(const MT& M, const DState& S, Env& E, int nr, const Real* w
       isc[12] = qi;
     });
     const int k = isc[10], q = isc[11], qi = isc[12];
+    // An island the register solver takes (k, q <= 16) is gathered into the
+    // LDS of the mass-factor slot, after the k x k Hessian factor that the
+    // solver keeps there, when it fits; otherwise into the HBM island scratch.
+    const bool lds_isl = k <= GL && q <= GL && 2 * k * k + q * k + 2 * q + 4 * k <= nv * nv;
+    if (lds_isl) {
+      sqm = LD(chol) + k * k;
+      sJ = sqm + k * k;
+      saref = sJ + q * k;
+      sdd = saref + q;
+      sa = sdd + q;
+      swarm = sa + k;
+      sqf = swarm + k;
+      swarm2 = sqf + k;
+    } else {
+      sqm = fs;
+      sJ = sqm + nv * nv;
+      saref = sJ + (int64_t)M.nrmax * nv;
+      sdd = saref + M.nrmax;
+      sa = sdd + M.nrmax;
+      swarm = sa + nv;
+      sqf = swarm + nv;
+      swarm2 = sqf + nv;
+    }
     X::par(k * k, [&](int t) { sqm[t] = LD(qm)[sidx[t / k] * nv + sidx[t % k]]; });
     X::par(q * k, [&](int t) { sJ[t] = J[sridx[t / k] * nv + sidx[t % k]]; });
     X::par(q, [&](int t) {
