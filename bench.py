@@ -173,19 +173,20 @@ def warm_start_study(model: str, n: int, sub: int, record: int, train_iters: int
             "kernel_ms_per_step_reference": kr / lr_, "kernel_ms_per_step_learned": km / lm}
 
 
-def pile_study(n: int, steps: int, seed: int, cpu_envs: int):
+def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = False):
     """Tabletop piles (pippin_amd/workloads.py random_pile_xml: 2 to 6 free
     primitives dropped onto a plane), after the reference's tumble scenes:
     env-steps/s on the GPU (4 physics steps per launch, bodies settling and
     resting) and of the oracle on the host cores over a sample of the same
-    envs and steps, for both solvers."""
+    envs and steps, for both solvers. general=True lets cylinders meet every
+    body (the general convex narrow phase)."""
     import torch
     from oracle.pyoracle import OracleSim
     from pippin_amd.physics import HipPhysics, load_model
     from pippin_amd.workloads import random_pile_xml
 
     threads = len(os.sched_getaffinity(0))
-    arrays = load_model(random_pile_xml(seed), is_xml=True)
+    arrays = load_model(random_pile_xml(seed, general), is_xml=True)
     out = []
     for solver in ("newton", "pgs"):
         sim = HipPhysics(arrays, n, solver=solver)
@@ -211,7 +212,8 @@ def pile_study(n: int, steps: int, seed: int, cpu_envs: int):
         c0 = time.perf_counter()
         ref.step(0, m, steps, threads=threads)
         cdt = time.perf_counter() - c0
-        out.append({"workload": f"pile seed {seed} ({nb} bodies), {solver}", "envs": n, "steps": steps,
+        kind = ", cylinders meet every body" if general else ""
+        out.append({"workload": f"pile seed {seed} ({nb} bodies{kind}), {solver}", "envs": n, "steps": steps,
                     "gpu_env_steps_per_s": n * steps / dt, "kernel_ms_per_launch": kms / launches,
                     "cpu_env_steps_per_s": m * steps / cdt, "cpu_envs": m, "cpu_threads": threads,
                     "newton_iters_mean": float(sim.solver_iterations().mean()),
@@ -242,10 +244,13 @@ def main():
     ap.add_argument("--piles", action="store_true",
                     help="only run the tabletop pile study (8192 envs, both solvers, GPU and host oracle) "
                          "and print one JSON line per solver")
+    ap.add_argument("--piles-general", action="store_true",
+                    help="as --piles, on piles whose cylinders meet every body (general convex pairs)")
     args = ap.parse_args()
-    if args.piles:
-        for seed in (11, 29):
-            for line in pile_study(8192, 200, seed, 1024):
+    if args.piles or args.piles_general:
+        seeds = (13, 18) if args.piles_general else (11, 29)
+        for seed in seeds:
+            for line in pile_study(8192, 200, seed, 1024, general=args.piles_general):
                 print(json.dumps(line), flush=True)
         return
     if args.warm_start:

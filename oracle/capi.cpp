@@ -362,7 +362,8 @@ int orc_collide(int ta, const double* pa, const double* ma, const double* sa, in
     b.mat[k] = mb[k];
   }
   std::vector<Hit> hits;
-  if (!needs_general(ta, tb) && pair_supported(ta, tb)) collide_pair(a, b, margin, hits);
+  if (needs_general(ta, tb)) collide_general(a, b, margin, hits);
+  else if (pair_supported(ta, tb)) collide_pair(a, b, margin, hits);
   for (size_t i = 0; i < hits.size() && (int32_t)i < cap; i++) {
     for (int k = 0; k < 3; k++) {
       out[i].pos[k] = hits[i].pos[k];

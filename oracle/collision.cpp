@@ -273,7 +273,7 @@ static std::vector<Vec3> clip(const std::vector<Vec3>& poly, Vec3 pn, Real off) 
 }
 
 // collision.rs:514-531. Rust's min_by keeps the first minimum, max_by the last maximum.
-static void reduce_to_four(std::vector<std::pair<Real, Vec3>>& pts) {
+void reduce_to_four(std::vector<std::pair<Real, Vec3>>& pts) {
   if (pts.size() <= 4) return;
   std::vector<std::pair<Real, Vec3>> keep;
   size_t di = 0;

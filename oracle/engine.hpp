@@ -194,6 +194,10 @@ void integrate_pos(const Model& m, Real* qpos, const Real* qvel, Real h);
 void collide_pair(const GeomPose& a, const GeomPose& b, Real margin, std::vector<Hit>& out);
 bool pair_supported(int a, int b);
 bool needs_general(int a, int b);
+// General convex pairs (collision.rs:124-147; parry's method restated in general.cpp).
+void collide_general(const GeomPose& a, const GeomPose& b, Real margin, std::vector<Hit>& out);
+// Keeps the deepest point and three spread around it (collision.rs:514-531).
+void reduce_to_four(std::vector<std::pair<Real, Vec3>>& pts);
 void contact_frame(const Contact& c, Vec3* t1, Vec3* t2);
 
 // ---- newton.rs / solver.rs ----

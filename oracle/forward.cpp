@@ -218,9 +218,8 @@ void collide(const Model& m, Data& d) {
     GeomPose p1 = {m.geom_type[g1], d.geom_xpos[g1], d.geom_xmat[g1], m.geom_size[g1]};
     GeomPose p2 = {m.geom_type[g2], d.geom_xpos[g2], d.geom_xmat[g2], m.geom_size[g2]};
     hits.clear();
-    // Mesh and cylinder-vs-non-plane pairs go to parry in the reference
-    // (collision.rs:102-147); that arithmetic is not restated here.
-    if (!needs_general(m.geom_type[g1], m.geom_type[g2])) collide_pair(p1, p2, margin, hits);
+    if (needs_general(m.geom_type[g1], m.geom_type[g2])) collide_general(p1, p2, margin, hits);
+    else collide_pair(p1, p2, margin, hits);
     for (size_t i = 0; i < hits.size(); i++) {
       Contact c;
       c.pos = hits[i].pos;

@@ -498,6 +498,7 @@ PHD void collide_pair(const GPose& a, const GPose& b, Real margin, Sink& out) {
 
 // Upper bound on the hits a pair of these types produces (used to size buffers).
 inline int pair_max_hits(int a, int b) {
+  if (a == G_MESH || b == G_MESH || (a == G_CYLINDER && b != G_PLANE) || (b == G_CYLINDER && a != G_PLANE)) return 4;
   if (a > b) {
     int t = a;
     a = b;

@@ -1,0 +1,695 @@
+// General convex narrow phase on the device: the pairs that
+// crates/pippin/src/collision.rs:102-107 (needs_general) sends to parry3d-f64
+// 0.31.1 (collide_general, collision.rs:124-147). Supported: a cylinder against a
+// sphere, capsule, box or cylinder.
+//
+// parry's published method, written for one lane per pair with fixed-size
+// private arrays:
+//   1. GJK distance between the convex cores (sphere = centre, capsule = segment,
+//      box, cylinder); sphere and capsule radii are added afterwards;
+//   2. EPA for depth and normal when the cores overlap (at most 32 vertices, 64
+//      faces, 28 expansions, stop when the support gains <= 1e-9);
+//   3. the support feature of each shape along the normal (box: most aligned
+//      face; cylinder: cap as a square when |n_axis| >= 0.5, else the side
+//      segment; capsule: its segment; sphere: its centre), overlapped in the
+//      plane normal to the contact normal, each point lifted back onto both
+//      features;
+//   4. points with separation below the margin are hits, more than four are
+//      reduced with the box-box rule (collision.rs:514-531), and the GJK/EPA
+//      witness pair is the hit when the features give none.
+// Hits: midpoint of the two surface points, normal from the first geom to the
+// second, depth = -separation (collision.rs:139-146). Every operation is in the
+// order of the CPU restatement that the tests compare against, bit for bit.
+#pragma once
+
+#include "dcollide.hpp"
+
+namespace phk {
+namespace gen {
+
+constexpr int GJK_ITERS = 64;
+constexpr int EPA_ITERS = 28;
+constexpr int EPA_V = 32;
+constexpr int EPA_F = 64;
+constexpr int EPA_E = 64;
+constexpr int GEN_CAP = 16;
+constexpr double EPA_TOL = 1e-9;
+
+struct Core {
+  int typ;
+  V3 pos;
+  M3 mat;
+  V3 size;
+  Real rad;
+};
+
+PHD Real vget(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+PHD void vset(V3& v, int k, Real x) {
+  if (k == 0) v.x = x;
+  else if (k == 1) v.y = x;
+  else v.z = x;
+}
+
+PHD Core core_of(const GPose& g) {
+  Core c;
+  c.typ = g.typ;
+  c.pos = g.pos;
+  c.mat = g.mat;
+  c.size = g.size;
+  c.rad = (g.typ == G_SPHERE || g.typ == G_CAPSULE) ? g.size.x : 0.0;
+  return c;
+}
+
+PHD V3 support_local(const Core& c, V3 d) {
+  if (c.typ == G_SPHERE) return v3z();
+  if (c.typ == G_CAPSULE) return v3(0.0, 0.0, d.z >= 0.0 ? c.size.y : -c.size.y);
+  if (c.typ == G_BOX)
+    return v3(d.x >= 0.0 ? c.size.x : -c.size.x, d.y >= 0.0 ? c.size.y : -c.size.y,
+              d.z >= 0.0 ? c.size.z : -c.size.z);
+  Real z = d.z >= 0.0 ? c.size.y : -c.size.y;
+  Real rho2 = d.x * d.x + d.y * d.y;
+  if (rho2 > 1e-30) {
+    Real s = c.size.x / sqrt_cr(rho2);
+    return v3(d.x * s, d.y * s, z);
+  }
+  return v3(0.0, 0.0, z);
+}
+
+PHD V3 support(const Core& c, V3 d) { return c.pos + mul_vec(c.mat, support_local(c, tmul_vec(c.mat, d))); }
+
+struct SV {
+  V3 w, a, b;
+};
+
+PHD SV msup(const Core& A, const Core& B, V3 d) {
+  SV s;
+  s.a = support(A, d);
+  s.b = support(B, -d);
+  s.w = s.a - s.b;
+  return s;
+}
+
+struct Simplex {
+  SV p[4];
+  Real lam[4];
+  int n;
+};
+
+PHD V3 simplex_point(const Simplex& s) {
+  V3 v = s.p[0].w * s.lam[0];
+  for (int i = 1; i < s.n; i++) v = v + s.p[i].w * s.lam[i];
+  return v;
+}
+
+PHD void keep1(Simplex& s, int i) {
+  s.p[0] = s.p[i];
+  s.lam[0] = 1.0;
+  s.n = 1;
+}
+
+PHD void keep2(Simplex& s, int i, int j, Real t) {
+  SV a = s.p[i], b = s.p[j];
+  s.p[0] = a;
+  s.p[1] = b;
+  s.lam[0] = 1.0 - t;
+  s.lam[1] = t;
+  s.n = 2;
+}
+
+PHD void closest_segment(Simplex& s) {
+  V3 a = s.p[0].w, ab = s.p[1].w - s.p[0].w;
+  Real t = -dot(a, ab);
+  if (t <= 0.0) {
+    keep1(s, 0);
+    return;
+  }
+  Real den = norm2(ab);
+  if (t >= den) {
+    keep1(s, 1);
+    return;
+  }
+  keep2(s, 0, 1, t / den);
+}
+
+// Voronoi regions of triangle (i, j, k) (Ericson, Real-Time Collision Detection,
+// 5.1.5) with the origin as the query point.
+PHD Real closest_triangle(const Simplex& s, int i, int j, int k, Simplex& r) {
+  V3 a = s.p[i].w, b = s.p[j].w, c = s.p[k].w;
+  V3 ab = b - a, ac = c - a;
+  Real d1 = -dot(ab, a), d2 = -dot(ac, a);
+  r = s;
+  if (d1 <= 0.0 && d2 <= 0.0) {
+    keep1(r, i);
+    return norm2(a);
+  }
+  Real d3 = -dot(ab, b), d4 = -dot(ac, b);
+  if (d3 >= 0.0 && d4 <= d3) {
+    keep1(r, j);
+    return norm2(b);
+  }
+  Real vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+    keep2(r, i, j, d1 / (d1 - d3));
+    return norm2(simplex_point(r));
+  }
+  Real d5 = -dot(ab, c), d6 = -dot(ac, c);
+  if (d6 >= 0.0 && d5 <= d6) {
+    keep1(r, k);
+    return norm2(c);
+  }
+  Real vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+    keep2(r, i, k, d2 / (d2 - d6));
+    return norm2(simplex_point(r));
+  }
+  Real va = d3 * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+    keep2(r, j, k, (d4 - d3) / ((d4 - d3) + (d5 - d6)));
+    return norm2(simplex_point(r));
+  }
+  Real den = 1.0 / (va + vb + vc);
+  Real v = vb * den, w = vc * den;
+  SV pa = s.p[i], pb = s.p[j], pc = s.p[k];
+  r.p[0] = pa;
+  r.p[1] = pb;
+  r.p[2] = pc;
+  r.lam[0] = 1.0 - v - w;
+  r.lam[1] = v;
+  r.lam[2] = w;
+  r.n = 3;
+  return norm2(simplex_point(r));
+}
+
+// 1: origin inside the tetrahedron; -1: flat tetrahedron; 0: reduced.
+PHD int closest(Simplex& s) {
+  if (s.n == 2) {
+    closest_segment(s);
+    return 0;
+  }
+  if (s.n == 3) {
+    Simplex r;
+    closest_triangle(s, 0, 1, 2, r);
+    s = r;
+    return 0;
+  }
+  const int F[4][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 3, 2, 0}};
+  V3 a = s.p[0].w;
+  Real vol = dot(s.p[3].w - a, cross(s.p[1].w - a, s.p[2].w - a));
+  if (fabs(vol) <= 1e-30) return -1;
+  bool any = false;
+  Real best = 0.0;
+  Simplex bs;
+  for (int f = 0; f < 4; f++) {
+    V3 p0 = s.p[F[f][0]].w;
+    V3 nrm = cross(s.p[F[f][1]].w - p0, s.p[F[f][2]].w - p0);
+    Real so = -dot(p0, nrm), sd = dot(s.p[F[f][3]].w - p0, nrm);
+    if (!(so * sd < 0.0)) continue;
+    Simplex r;
+    Real d = closest_triangle(s, F[f][0], F[f][1], F[f][2], r);
+    if (!any || d < best) {
+      any = true;
+      best = d;
+      bs = r;
+    }
+  }
+  if (!any) return 1;
+  s = bs;
+  return 0;
+}
+
+struct Face {
+  int i[3];
+  V3 n;
+  Real d;
+};
+
+PHD bool make_face(Face& f, const SV* V, int a, int b, int c) {
+  V3 nrm = cross(V[b].w - V[a].w, V[c].w - V[a].w);
+  Real l = norm(nrm);
+  if (!(l > 1e-30)) return false;
+  f.i[0] = a;
+  f.i[1] = b;
+  f.i[2] = c;
+  f.n = nrm * (1.0 / l);
+  f.d = dot(f.n, V[a].w);
+  return true;
+}
+
+struct Penetration {
+  V3 n, pa, pb;
+  Real depth;
+};
+
+PHD Penetration face_result(const Face& f, const SV* V) {
+  Penetration r;
+  r.n = f.n;
+  r.depth = f.d;
+  V3 a = V[f.i[0]].w;
+  V3 v0 = V[f.i[1]].w - a, v1 = V[f.i[2]].w - a, v2 = f.n * f.d - a;
+  Real d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+  Real den = d00 * d11 - d01 * d01;
+  Real u = 1.0, v = 0.0, w = 0.0;
+  if (fabs(den) > 1e-30) {
+    v = (d11 * d20 - d01 * d21) / den;
+    w = (d00 * d21 - d01 * d20) / den;
+    u = 1.0 - v - w;
+  }
+  r.pa = V[f.i[0]].a * u + V[f.i[1]].a * v + V[f.i[2]].a * w;
+  r.pb = V[f.i[0]].b * u + V[f.i[1]].b * v + V[f.i[2]].b * w;
+  return r;
+}
+
+PHD V3 axis_dir(int k) {
+  // +x, -x, +y, -y, +z, -z
+  Real s = (k & 1) ? -1.0 : 1.0;
+  int a = k >> 1;
+  return v3(a == 0 ? s : 0.0, a == 1 ? s : 0.0, a == 2 ? s : 0.0);
+}
+
+PHD bool epa(const Core& A, const Core& B, const Simplex& s, Penetration* out) {
+  SV V[EPA_V];
+  int nv = s.n;
+  for (int k = 0; k < nv; k++) V[k] = s.p[k];
+  if (nv == 1) {
+    for (int k = 0; k < 6 && nv == 1; k++) {
+      SV w = msup(A, B, axis_dir(k));
+      if (norm2(w.w - V[0].w) > 1e-18) V[nv++] = w;
+    }
+  }
+  if (nv == 2) {
+    V3 e = V[1].w - V[0].w, ea = vabs(e);
+    int k = 0;
+    if (ea.y < vget(ea, k)) k = 1;
+    if (ea.z < vget(ea, k)) k = 2;
+    V3 d = cross(e, axis_dir(2 * k));
+    for (int sg = 0; sg < 2 && nv == 2; sg++) {
+      SV w = msup(A, B, sg == 0 ? d : -d);
+      if (norm2(cross(w.w - V[0].w, e)) > 1e-18 * norm2(e)) V[nv++] = w;
+    }
+  }
+  if (nv == 3) {
+    V3 nrm = cross(V[1].w - V[0].w, V[2].w - V[0].w);
+    for (int sg = 0; sg < 2 && nv == 3; sg++) {
+      SV w = msup(A, B, sg == 0 ? nrm : -nrm);
+      if (fabs(dot(w.w - V[0].w, nrm)) > 1e-9 * norm(nrm)) V[nv++] = w;
+    }
+  }
+  if (nv < 4) return false;
+  if (dot(cross(V[1].w - V[0].w, V[2].w - V[0].w), V[3].w - V[0].w) > 0.0) {
+    SV t = V[1];
+    V[1] = V[2];
+    V[2] = t;
+  }
+  Face F[EPA_F];
+  int nf = 0;
+  const int T[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+  for (int k = 0; k < 4; k++)
+    if (!make_face(F[nf++], V, T[k][0], T[k][1], T[k][2])) return false;
+  int E[EPA_E][2];
+  for (int it = 0; it < EPA_ITERS; it++) {
+    int bi = 0;
+    for (int k = 1; k < nf; k++)
+      if (F[k].d < F[bi].d) bi = k;
+    *out = face_result(F[bi], V);
+    if (nv == EPA_V) return true;
+    SV w = msup(A, B, F[bi].n);
+    if (dot(w.w, F[bi].n) - F[bi].d <= EPA_TOL) return true;
+    int ne = 0;
+    for (int k = 0; k < nf;) {
+      if (dot(F[k].n, w.w - V[F[k].i[0]].w) > 0.0) {
+        for (int q = 0; q < 3; q++) {
+          int ea = F[k].i[q], eb = F[k].i[(q + 1) % 3];
+          int found = -1;
+          for (int m = 0; m < ne; m++)
+            if (E[m][0] == eb && E[m][1] == ea) found = m;
+          if (found >= 0) {
+            E[found][0] = E[ne - 1][0];
+            E[found][1] = E[ne - 1][1];
+            ne--;
+          } else {
+            if (ne == EPA_E) return true;
+            E[ne][0] = ea;
+            E[ne][1] = eb;
+            ne++;
+          }
+        }
+        F[k] = F[nf - 1];
+        nf--;
+      } else {
+        k++;
+      }
+    }
+    V[nv] = w;
+    for (int m = 0; m < ne; m++) {
+      if (nf == EPA_F) return true;
+      if (!make_face(F[nf], V, E[m][0], E[m][1], nv)) return true;
+      nf++;
+    }
+    nv++;
+  }
+  int bi = 0;
+  for (int k = 1; k < nf; k++)
+    if (F[k].d < F[bi].d) bi = k;
+  *out = face_result(F[bi], V);
+  return true;
+}
+
+PHD int feature(const Core& c, V3 dir, V3* out) {
+  V3 d = tmul_vec(c.mat, dir);
+  V3 loc[4];
+  int n;
+  if (c.typ == G_SPHERE) {
+    loc[0] = v3z();
+    n = 1;
+  } else if (c.typ == G_CAPSULE) {
+    loc[0] = v3(0.0, 0.0, -c.size.y);
+    loc[1] = v3(0.0, 0.0, c.size.y);
+    n = 2;
+  } else if (c.typ == G_BOX) {
+    V3 da = vabs(d);
+    int k = 0;
+    if (da.y > vget(da, k)) k = 1;
+    if (da.z > vget(da, k)) k = 2;
+    int u = (k + 1) % 3, v = (k + 2) % 3;
+    Real su = vget(c.size, u), sv = vget(c.size, v);
+    const Real cu[4] = {su, -su, -su, su};
+    const Real cv[4] = {sv, sv, -sv, -sv};
+    Real sk = vget(c.size, k);
+    Real pk = vget(d, k) >= 0.0 ? sk : -sk;
+    for (int q = 0; q < 4; q++) {
+      V3 p = v3z();
+      vset(p, k, pk);
+      vset(p, u, cu[q]);
+      vset(p, v, cv[q]);
+      loc[q] = p;
+    }
+    n = 4;
+  } else {
+    Real r = c.size.x, h = c.size.y;
+    Real rho2 = d.x * d.x + d.y * d.y;
+    Real cx = 1.0, cy = 0.0;
+    if (rho2 > 1e-30) {
+      Real s = 1.0 / sqrt_cr(rho2);
+      cx = d.x * s;
+      cy = d.y * s;
+    }
+    if (fabs(d.z) < 0.5) {
+      loc[0] = v3(r * cx, r * cy, -h);
+      loc[1] = v3(r * cx, r * cy, h);
+      n = 2;
+    } else {
+      Real z = d.z >= 0.0 ? h : -h;
+      loc[0] = v3(r * cx, r * cy, z);
+      loc[1] = v3(-(r * cy), r * cx, z);
+      loc[2] = v3(-(r * cx), -(r * cy), z);
+      loc[3] = v3(r * cy, -(r * cx), z);
+      n = 4;
+    }
+  }
+  for (int q = 0; q < n; q++) out[q] = c.pos + mul_vec(c.mat, loc[q]);
+  return n;
+}
+
+struct P2 {
+  Real x, y;
+};
+
+PHD P2 proj2(V3 p, V3 t1, V3 t2) {
+  P2 r;
+  r.x = dot(p, t1);
+  r.y = dot(p, t2);
+  return r;
+}
+PHD Real cross2(P2 a, P2 b) { return a.x * b.y - a.y * b.x; }
+PHD P2 sub2(P2 a, P2 b) {
+  P2 r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+
+struct Cands {
+  Real dist[GEN_CAP];
+  V3 pos[GEN_CAP];
+  int n;
+};
+
+PHD void emit(Cands& c, V3 pa, V3 pb, V3 n, Real ra, Real rb, Real margin) {
+  Real dist = dot(pb - pa, n) - ra - rb;
+  if (dist < margin && c.n < GEN_CAP) {
+    c.dist[c.n] = dist;
+    c.pos[c.n] = ((pa + n * ra) + (pb - n * rb)) * 0.5;
+    c.n++;
+  }
+}
+
+PHD void seg_poly(const V3* s, const V3* p, int np, bool s_is_a, V3 n, V3 t1, V3 t2, Real ra, Real rb, Real margin,
+                  Cands& c) {
+  P2 s0 = proj2(s[0], t1, t2), s1 = proj2(s[1], t1, t2);
+  P2 e = sub2(s1, s0);
+  if (!(e.x * e.x + e.y * e.y > 1e-18)) return;
+  P2 q[4];
+  for (int k = 0; k < np; k++) q[k] = proj2(p[k], t1, t2);
+  Real area = 0.0;
+  for (int k = 0; k < np; k++) area = area + cross2(q[k], q[(k + 1) % np]);
+  Real orient = area >= 0.0 ? 1.0 : -1.0;
+  Real tlo = 0.0, thi = 1.0;
+  for (int k = 0; k < np; k++) {
+    P2 ed = sub2(q[(k + 1) % np], q[k]);
+    Real num = orient * cross2(ed, sub2(s0, q[k]));
+    Real den = orient * cross2(ed, e);
+    if (den == 0.0) {
+      if (num < 0.0) return;
+      continue;
+    }
+    Real t = -num / den;
+    if (den > 0.0) tlo = rmax(tlo, t);
+    else thi = rmin(thi, t);
+  }
+  if (tlo > thi) return;
+  V3 np3 = cross(p[1] - p[0], p[2] - p[0]);
+  Real den = dot(n, np3);
+  if (!(fabs(den) > 1e-12 * norm(np3))) return;
+  int m = thi - tlo > 1e-12 ? 2 : 1;
+  for (int k = 0; k < m; k++) {
+    Real t = k == 0 ? tlo : thi;
+    V3 ps = s[0] + (s[1] - s[0]) * t;
+    V3 pp = ps + n * (dot(p[0] - ps, np3) / den);
+    if (s_is_a) emit(c, ps, pp, n, ra, rb, margin);
+    else emit(c, pp, ps, n, ra, rb, margin);
+  }
+}
+
+PHD void seg_seg(const V3* a, const V3* b, V3 n, V3 t1, V3 t2, Real ra, Real rb, Real margin, Cands& c) {
+  P2 a0 = proj2(a[0], t1, t2), a1 = proj2(a[1], t1, t2), b0 = proj2(b[0], t1, t2), b1 = proj2(b[1], t1, t2);
+  P2 e = sub2(a1, a0), f = sub2(b1, b0);
+  Real ee = e.x * e.x + e.y * e.y, ff = f.x * f.x + f.y * f.y;
+  if (!(ee > 1e-18) || !(ff > 1e-18)) return;
+  Real cr = cross2(e, f);
+  if (cr * cr > 1e-12 * ee * ff) return;
+  P2 g0 = sub2(b0, a0), g1 = sub2(b1, a0);
+  Real u0 = (g0.x * e.x + g0.y * e.y) / ee, u1 = (g1.x * e.x + g1.y * e.y) / ee;
+  Real lo = rmax(0.0, rmin(u0, u1)), hi = rmin(1.0, rmax(u0, u1));
+  if (lo > hi) return;
+  int m = hi - lo > 1e-12 ? 2 : 1;
+  for (int k = 0; k < m; k++) {
+    Real t = k == 0 ? lo : hi;
+    V3 pa = a[0] + (a[1] - a[0]) * t;
+    P2 q = proj2(pa, t1, t2);
+    P2 h = sub2(q, b0);
+    Real s = (h.x * f.x + h.y * f.y) / ff;
+    V3 pb = b[0] + (b[1] - b[0]) * s;
+    emit(c, pa, pb, n, ra, rb, margin);
+  }
+}
+
+PHD void poly_poly(const V3* a, int na, const V3* b, int nb, V3 n, V3 t1, V3 t2, Real ra, Real rb, Real margin,
+                   Cands& c) {
+  P2 qa[4], buf[2][GEN_CAP];
+  for (int k = 0; k < na; k++) qa[k] = proj2(a[k], t1, t2);
+  int m = nb;
+  for (int k = 0; k < nb; k++) buf[0][k] = proj2(b[k], t1, t2);
+  Real area = 0.0;
+  for (int k = 0; k < na; k++) area = area + cross2(qa[k], qa[(k + 1) % na]);
+  Real orient = area >= 0.0 ? 1.0 : -1.0;
+  int cur = 0;
+  for (int k = 0; k < na && m > 0; k++) {
+    P2 p0 = qa[k], ed = sub2(qa[(k + 1) % na], qa[k]);
+    const P2* in = buf[cur];
+    P2* o = buf[1 - cur];
+    int mo = 0;
+    for (int q = 0; q < m; q++) {
+      P2 s = in[(q + m - 1) % m], e = in[q];
+      Real ds = orient * cross2(ed, sub2(s, p0)), de = orient * cross2(ed, sub2(e, p0));
+      if (de >= 0.0) {
+        if (ds < 0.0 && mo < GEN_CAP) {
+          Real t = ds / (ds - de);
+          P2 x;
+          x.x = s.x + (e.x - s.x) * t;
+          x.y = s.y + (e.y - s.y) * t;
+          o[mo++] = x;
+        }
+        if (mo < GEN_CAP) o[mo++] = e;
+      } else if (ds >= 0.0 && mo < GEN_CAP) {
+        Real t = ds / (ds - de);
+        P2 x;
+        x.x = s.x + (e.x - s.x) * t;
+        x.y = s.y + (e.y - s.y) * t;
+        o[mo++] = x;
+      }
+    }
+    m = mo;
+    cur = 1 - cur;
+  }
+  if (m == 0) return;
+  V3 nA = cross(a[1] - a[0], a[2] - a[0]), nB = cross(b[1] - b[0], b[2] - b[0]);
+  Real dA = dot(n, nA), dB = dot(n, nB);
+  if (!(fabs(dA) > 1e-12 * norm(nA)) || !(fabs(dB) > 1e-12 * norm(nB))) return;
+  for (int k = 0; k < m; k++) {
+    V3 x = t1 * buf[cur][k].x + t2 * buf[cur][k].y;
+    V3 pa = x + n * (dot(a[0] - x, nA) / dA);
+    V3 pb = x + n * (dot(b[0] - x, nB) / dB);
+    emit(c, pa, pb, n, ra, rb, margin);
+  }
+}
+
+// The deepest point, the farthest from it, the one spanning the largest
+// triangle with those, and the one farthest from all three (collision.rs:514-531).
+PHD void reduce4(Cands& c) {
+  if (c.n <= 4) return;
+  int npt = c.n;
+  int keep[4];
+  int di = 0;
+  for (int q = 1; q < npt; q++)
+    if (c.dist[q] < c.dist[di]) di = q;
+  keep[0] = di;
+  V3 deep = c.pos[di];
+  int fa = 0;
+  Real fk = norm2(c.pos[0] - deep);
+  for (int q = 1; q < npt; q++) {
+    Real kk = norm2(c.pos[q] - deep);
+    if (!(kk < fk)) {
+      fk = kk;
+      fa = q;
+    }
+  }
+  keep[1] = fa;
+  V3 e01 = c.pos[keep[1]] - c.pos[keep[0]];
+  int th = 0;
+  Real tk = norm(cross(e01, c.pos[0] - c.pos[keep[0]]));
+  for (int q = 1; q < npt; q++) {
+    Real kk = norm(cross(e01, c.pos[q] - c.pos[keep[0]]));
+    if (!(kk < tk)) {
+      tk = kk;
+      th = q;
+    }
+  }
+  keep[2] = th;
+  int fo = 0;
+  Real fok = 0;
+  for (int q = 0; q < npt; q++) {
+    Real sprd = INFINITY;
+    for (int t = 0; t < 3; t++) sprd = rmin(sprd, norm(c.pos[keep[t]] - c.pos[q]));
+    if (q == 0 || !(sprd < fok)) {
+      fok = sprd;
+      fo = q;
+    }
+  }
+  keep[3] = fo;
+  Real d4[4];
+  V3 p4[4];
+  for (int t = 0; t < 4; t++) {
+    d4[t] = c.dist[keep[t]];
+    p4[t] = c.pos[keep[t]];
+  }
+  for (int t = 0; t < 4; t++) {
+    c.dist[t] = d4[t];
+    c.pos[t] = p4[t];
+  }
+  c.n = 4;
+}
+
+}  // namespace gen
+
+PHD bool needs_general(int a, int b) {
+  return a == G_MESH || b == G_MESH || (a == G_CYLINDER && b != G_PLANE) || (b == G_CYLINDER && a != G_PLANE);
+}
+
+// collision.rs:124-147, the method described at the top of this file.
+inline __host__ __device__ __noinline__ void collide_general(const GPose& ga, const GPose& gb, Real margin, Sink& out) {
+  using namespace gen;
+  if (ga.typ == G_MESH || gb.typ == G_MESH || ga.typ == G_PLANE || gb.typ == G_PLANE) return;
+  Core A = core_of(ga), B = core_of(gb);
+  Simplex s;
+  V3 d0 = B.pos - A.pos;
+  if (!(norm2(d0) > 1e-30)) d0 = v3(1.0, 0.0, 0.0);
+  s.p[0] = msup(A, B, d0);
+  s.lam[0] = 1.0;
+  s.n = 1;
+  V3 v = s.p[0].w;
+  Real vv = norm2(v);
+  bool overlap = false;
+  for (int it = 0; it < GJK_ITERS; it++) {
+    if (vv <= 1e-24) {
+      overlap = true;
+      break;
+    }
+    SV w = msup(A, B, -v);
+    if (vv - dot(v, w.w) <= 1e-12 * vv) break;
+    Simplex prev = s;
+    s.p[s.n] = w;
+    s.n++;
+    int st = closest(s);
+    if (st == 1) {
+      overlap = true;
+      break;
+    }
+    if (st == -1) {
+      s = prev;
+      break;
+    }
+    V3 vn = simplex_point(s);
+    Real vvn = norm2(vn);
+    if (!(vvn < vv)) {
+      s = prev;
+      break;
+    }
+    v = vn;
+    vv = vvn;
+  }
+  V3 n, wa, wb;
+  Real dcore;
+  if (overlap) {
+    Penetration p;
+    if (!epa(A, B, s, &p)) return;
+    n = p.n;
+    dcore = -p.depth;
+    wa = p.pa;
+    wb = p.pb;
+  } else {
+    dcore = sqrt_cr(vv);
+    n = v * (-1.0 / dcore);
+    wa = s.p[0].a * s.lam[0];
+    wb = s.p[0].b * s.lam[0];
+    for (int i = 1; i < s.n; i++) {
+      wa = wa + s.p[i].a * s.lam[i];
+      wb = wb + s.p[i].b * s.lam[i];
+    }
+  }
+  if (!(dcore - A.rad - B.rad < margin)) return;
+  V3 fa[4], fb[4];
+  int na = feature(A, n, fa), nb = feature(B, -n, fb);
+  V3 t1, t2;
+  tangents(n, &t1, &t2);
+  Cands c;
+  c.n = 0;
+  if (na == 2 && nb == 2) seg_seg(fa, fb, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na == 2 && nb == 4) seg_poly(fa, fb, nb, true, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na == 4 && nb == 2) seg_poly(fb, fa, na, false, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na == 4 && nb == 4) poly_poly(fa, na, fb, nb, n, t1, t2, A.rad, B.rad, margin, c);
+  if (c.n == 0) emit(c, wa, wb, n, A.rad, B.rad, margin);
+  reduce4(c);
+  for (int k = 0; k < c.n; k++) out.push(c.pos[k], n, -c.dist[k]);
+}
+
+}  // namespace phk

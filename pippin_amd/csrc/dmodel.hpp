@@ -89,6 +89,7 @@ inline bool layout_equal(const Layout& a, const Layout& b) { return std::memcmp(
 
 struct DModel {
   int nq, nv, nu, nbody, njnt, ngeom, ncam, neq, npair, nlevel, nblock, nacc;
+  int ngeneral;  // candidate pairs that take the general convex narrow phase (dgeneral.hpp)
   int integrator, solver_kind, iterations;
   Real timestep, gravity[3], tolerance, margin, solref[2], solimp[5], impratio, hertz, zeta, slop;
   int ncmax;  // contact capacity per env (the sum of per-pair maxima: never exceeded)

@@ -118,6 +118,7 @@ static bool needs_general(int a, int b) {
 }
 
 static int pair_max_hits(int a, int b) {
+  if (needs_general(a, b)) return 4;  // manifolds above four points are reduced to four
   if (a > b) std::swap(a, b);
   if (a == PH_GEOM_PLANE && b == PH_GEOM_CAPSULE) return 2;
   if ((a == PH_GEOM_PLANE && (b == PH_GEOM_BOX || b == PH_GEOM_CYLINDER)) || (a == PH_GEOM_BOX && b == PH_GEOM_BOX))
@@ -319,25 +320,19 @@ bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
   // candidate pairs in lexicographic order (model.rs:398-407)
   m.pair_g1.clear();
   m.pair_g2.clear();
-  m.general_g1.clear();
-  m.general_g2.clear();
+  m.ngeneral = 0;
   m.ncmax = 0;
   for (int g1 = 0; g1 < ng; g1++)
     for (int g2 = g1 + 1; g2 < ng; g2++) {
       if (!pair_static_ok(m, g1, g2)) continue;
       if (needs_general(m.geom_type[g1], m.geom_type[g2])) {
-        // Mesh and cylinder-vs-non-plane pairs go through parry in the
-        // reference (collision.rs:102-147), which the device does not run.
-        // Such a pair is allowed only while the collision masks keep it off;
-        // it is listed so that a mask change enabling it can be refused.
-        bool on = (m.geom_contype[g1] & m.geom_conaffinity[g2]) || (m.geom_contype[g2] & m.geom_conaffinity[g1]);
-        if (on) {
-          *err = "mesh and cylinder-vs-non-plane pairs need the general convex narrow phase, not implemented on the device";
+        // cylinder-vs-non-plane pairs take the general convex narrow phase
+        // (dgeneral.hpp); mesh geoms have no hull on the device
+        if (m.geom_type[g1] == PH_GEOM_MESH || m.geom_type[g2] == PH_GEOM_MESH) {
+          *err = "mesh geoms need convex hulls, which this backend does not build";
           return false;
         }
-        m.general_g1.push_back(g1);
-        m.general_g2.push_back(g2);
-        continue;
+        m.ngeneral++;
       }
       m.pair_g1.push_back(g1);
       m.pair_g2.push_back(g2);
@@ -381,6 +376,7 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
   M.ncam = d.ncam;
   M.neq = d.neq;
   M.npair = (int)hm.pair_g1.size();
+  M.ngeneral = hm.ngeneral;
   M.nlevel = (int)hm.level_adr.size() - 1;
   M.nacc = (int)hm.acc_level_adr.size() - 1;
   M.nblock = hm.block_adr.empty() ? 0 : (int)hm.block_adr.size() - 1;

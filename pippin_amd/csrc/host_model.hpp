@@ -36,7 +36,7 @@ struct HostModel {
   std::vector<int32_t> acc_level_adr, acc_parent, acc_child_adr, acc_child;
   std::vector<double> geom_aabb;  // ngeom x 6 (center, half-extents)
   std::vector<int32_t> pair_g1, pair_g2;  // candidate pairs: every static filter except the masks
-  std::vector<int32_t> general_g1, general_g2;  // pairs only the general narrow phase handles (kept off by masks)
+  int ngeneral = 0;  // candidate pairs that take the general convex narrow phase
   std::vector<int32_t> dofact_adr, dofact_num, dofact_u;
   std::vector<double> dofact_coef;
   double contact_hertz = 0;

@@ -12,7 +12,7 @@
 // dofs, pairs, matrix entries, Cholesky rows), so each lane performs exactly
 // the reference's sequence of operations for the values it owns.
 #pragma once
-#include "dcollide.hpp"
+#include "dgeneral.hpp"
 #include "lanes.hpp"
 
 namespace phk {
@@ -424,7 +424,8 @@ PHD void collide(const MT& M, const DState& S, Env& E) {
     Sink sink;
     sink.out = stage + (int64_t)p * MAXHIT * CW;
     sink.n = 0;
-    collide_pair(a, b, margin, sink);
+    if (M.ngeneral > 0 && needs_general(a.typ, b.typ)) collide_general(a, b, margin, sink);
+    else collide_pair(a, b, margin, sink);
     cnt[p] = sink.n;
   });
   // pair-ordered compaction (contacts are listed in pair order, then hit

@@ -799,18 +799,6 @@ static int param_apply(ph_sim* s, int v, int p, int64_t id, const double* x, boo
     case PH_PARAM_GEOM_CONTYPE:
     case PH_PARAM_GEOM_CONAFFINITY: {
       uint32_t bits = sat_u32(std::fmax(x[0], 0.0));
-      // refuse a mask that would enable a pair only the general narrow phase handles
-      const uint32_t* ct = &V.contype[(int64_t)v * d.ngeom];
-      const uint32_t* ca = &V.conaffinity[(int64_t)v * d.ngeom];
-      for (size_t k = 0; k < s->hm.general_g1.size(); k++) {
-        int g1 = s->hm.general_g1[k], g2 = s->hm.general_g2[k];
-        auto t = [&](int g) { return (p == PH_PARAM_GEOM_CONTYPE && g == id) ? bits : ct[g]; };
-        auto a = [&](int g) { return (p == PH_PARAM_GEOM_CONAFFINITY && g == id) ? bits : ca[g]; };
-        if ((t(g1) & a(g2)) || (t(g2) & a(g1)))
-          return fail(PH_ERR_ARG, std::string(param_name(p)) +
-                                      ": would enable a mesh or cylinder pair, which needs the general narrow phase "
-                                      "(not on the device)");
-      }
       if (apply) (p == PH_PARAM_GEOM_CONTYPE ? V.contype : V.conaffinity)[(int64_t)v * d.ngeom + id] = bits;
       break;
     }

@@ -2,11 +2,12 @@
 from __future__ import annotations
 
 
-def random_pile_xml(seed: int) -> str:
+def random_pile_xml(seed: int, general: bool = False) -> str:
     """2 to 6 free primitives (box, sphere, capsule, cylinder) of random sizes
     and orientations dropped in a column onto a plane, after the reference's
-    tabletop tumble scenes. Cylinders collide with the plane only (their
-    other pairs would need the general narrow phase)."""
+    tabletop tumble scenes. With general=False cylinders collide with the plane
+    only; with general=True they also meet every other body, through the
+    general convex narrow phase."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -24,8 +25,9 @@ def random_pile_xml(seed: int) -> str:
         elif t == "capsule":
             g = '<geom type="capsule" size="%.4f %.4f"/>' % (rng.uniform(0.015, 0.03), rng.uniform(0.02, 0.05))
         else:
-            g = ('<geom type="cylinder" size="%.4f %.4f" contype="2" conaffinity="0"/>'
-                 % (rng.uniform(0.015, 0.03), rng.uniform(0.02, 0.05)))
+            mask = "" if general else ' contype="2" conaffinity="0"'
+            g = ('<geom type="cylinder" size="%.4f %.4f"%s/>'
+                 % (rng.uniform(0.015, 0.03), rng.uniform(0.02, 0.05), mask))
         bodies.append('<body pos="%.4f %.4f %.4f" euler="%.1f %.1f %.1f"><freejoint/>%s</body>'
                       % (x, y, z, e[0], e[1], e[2], g))
     return ('<mujoco><option timestep="0.004"/><worldbody>'

@@ -15,6 +15,29 @@ PRIMITIVE_SCENES = {
 }
 
 
+# pairs of the general convex narrow phase (a cylinder against a non-plane):
+# oracle/general.cpp against pippin_amd/csrc/dgeneral.hpp
+GENERAL_SCENES = {
+    # crates/pippin/tests/mesh.rs:66-83: upright, on its side, and stacked
+    "cylinders_on_box": '<geom type="box" size="1 1 .1" pos="0 0 -.1"/>'
+                        '<body name="up" pos="0 0 0.3"><freejoint/><geom type="cylinder" size=".04 .06"/></body>'
+                        '<body name="side" pos="0.5 0 0.3" euler="90 0 0"><freejoint/><geom type="cylinder" size=".04 .06"/></body>'
+                        '<body name="onup" pos="0 0 0.5"><freejoint/><geom type="cylinder" size=".03 .03"/></body>',
+    "sphere_on_cylinder": '<geom type="plane" size="5 5 .1"/><geom type="cylinder" size=".05 .05" pos="0 0 .05"/>'
+                          '<body pos=".03 .01 .2"><freejoint/><geom type="sphere" size=".03"/></body>',
+    "capsule_on_cylinder": '<geom type="plane" size="5 5 .1"/><geom type="cylinder" size=".04 .1" pos="0 0 .04" euler="90 0 0"/>'
+                           '<body pos=".01 0 .2" euler="0 80 20"><freejoint/><geom type="capsule" size=".02 .06"/></body>',
+    "cylinder_on_cylinder": '<geom type="plane" size="5 5 .1"/><geom type="cylinder" size=".04 .1" pos="0 0 .04" euler="90 0 0"/>'
+                            '<body pos="0 .02 .2" euler="10 70 5"><freejoint/><geom type="cylinder" size=".03 .05"/></body>',
+    "box_on_cylinder": '<geom type="plane" size="5 5 .1"/><geom type="cylinder" size=".06 .05" pos="0 0 .05"/>'
+                       '<body pos=".01 0 .2" euler="5 8 30"><freejoint/><geom type="box" size=".04 .03 .02"/></body>',
+}
+
+
+def general_xml(name: str) -> str:
+    return f'<mujoco><option timestep="0.004"/><worldbody>{GENERAL_SCENES[name]}</worldbody></mujoco>'
+
+
 def primitive_xml(name: str) -> str:
     return f'<mujoco><option timestep="0.004"/><worldbody>{PRIMITIVE_SCENES[name]}</worldbody></mujoco>'
 

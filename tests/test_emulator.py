@@ -13,7 +13,7 @@ import pytest
 from oracle import mjcf
 from oracle.pyoracle import OracleSim
 from tests.emu.pyemu import EmuSim
-from tests.scenes import MULTIBODY_SCENES, PRIMITIVE_SCENES, primitive_xml
+from tests.scenes import GENERAL_SCENES, MULTIBODY_SCENES, PRIMITIVE_SCENES, general_xml, primitive_xml
 
 
 def _perturbed(q, rng, scale, free=True):
@@ -322,6 +322,48 @@ def test_random_piles_exact(seed, solver):
     from tests.scenes import random_pile_xml
 
     arr = mjcf.load_string(random_pile_xml(seed))
+    arr["solver_kind"] = solver
+    n = 3
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    for t in range(200):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("name", sorted(GENERAL_SCENES))
+@pytest.mark.parametrize("solver", [0, 1])
+def test_general_pairs_exact(name, solver):
+    """The general convex narrow phase (dgeneral.hpp) against its CPU
+    restatement (oracle/general.cpp), Newton and PGS, bit for bit."""
+    arr = mjcf.load_string(general_xml(name))
+    arr["solver_kind"] = solver
+    n = 4
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(17)
+    q = o.get("qpos")
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.01
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+    for t in range(250):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert np.array_equal(o.solver_iterations(), e.solver_iterations())
+    assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("seed", [2, 13])
+@pytest.mark.parametrize("solver", [0, 1])
+def test_random_piles_with_general_pairs_exact(seed, solver):
+    """Random piles whose cylinders meet every body (general pairs)."""
+    from pippin_amd.workloads import random_pile_xml
+
+    arr = mjcf.load_string(random_pile_xml(seed, general=True))
     arr["solver_kind"] = solver
     n = 3
     o, e = OracleSim(arr, n), EmuSim(arr, n)

@@ -15,7 +15,7 @@ import pytest
 from oracle import mjcf
 from oracle.pyoracle import OracleSim, cr_sincos
 from pippin_amd.physics import HipPhysics, selftest_math
-from tests.scenes import MULTIBODY_SCENES, PRIMITIVE_SCENES, primitive_xml
+from tests.scenes import GENERAL_SCENES, MULTIBODY_SCENES, PRIMITIVE_SCENES, general_xml, primitive_xml
 
 pytestmark = pytest.mark.gpu
 
@@ -333,6 +333,57 @@ def test_random_piles_match(seed, solver):
     for s in (o, g):
         s.set("qpos", 0, n, q)
     say("random pile", seed, solver, n)
+    for t in range(250):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("name", sorted(GENERAL_SCENES))
+@pytest.mark.parametrize("solver", ["newton", "pgs"])
+def test_general_pairs_match(name, solver):
+    """Cylinder against sphere, capsule, box and cylinder through the general
+    convex narrow phase (GJK/EPA and feature clipping), against the oracle's
+    restatement of the same method."""
+    arr = mjcf.load_string(general_xml(name))
+    arr["solver_kind"] = 0 if solver == "newton" else 1
+    n = 16
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(19)
+    q = o.get("qpos")
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.01
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    say("general pairs", name, solver, n)
+    for t in range(250):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert np.array_equal(o.solver_iterations(), g.solver_iterations())
+    assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("seed", [2, 13, 18])
+@pytest.mark.parametrize("solver", ["newton", "pgs"])
+def test_random_piles_with_general_pairs_match(seed, solver):
+    """Random piles whose cylinders meet every other body."""
+    from pippin_amd.workloads import random_pile_xml
+
+    arr = mjcf.load_string(random_pile_xml(seed, general=True))
+    arr["solver_kind"] = 0 if solver == "newton" else 1
+    n = 24
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(seed)
+    q = o.get("qpos")
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    say("random pile with general pairs", seed, solver, n)
     for t in range(250):
         for s in (o, g):
             s.step(0, n, 1)

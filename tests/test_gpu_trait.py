@@ -204,17 +204,30 @@ def test_contact_free_trajectories(assets, name):
     assert err == 0.0
 
 
-def test_masks_cannot_enable_general_pairs(assets):
-    """arm3d's cylinder meets boxes and capsules only through the general convex
-    narrow phase, which the device does not run: the model is accepted while
-    its collision masks keep those pairs off, and a mask change that would turn
-    one on is refused without touching the stored masks."""
-    g = HipPhysics(mjcf.load_file(assets / "arm3d.xml"), 3)
-    g.set_param("geom_contype", 0, 0, 3, [1.0, 1.0, 1.0])  # no conaffinity anywhere: still off
-    with pytest.raises(ParamError, match="general"):
-        g.set_param("geom_conaffinity", 2, 1, 2, [1.0])  # cylinder (geom 0) x box (geom 2)
-    assert np.array_equal(g.get_param("geom_conaffinity", 2)[:, 0], [0, 0, 0])
-    assert np.array_equal(g.get_param("geom_contype", 0)[:, 0], [1, 1, 1])
+def test_masks_switch_general_pairs_on_per_env(assets):
+    """arm3d's cylinder meets its boxes and capsules only through the general
+    convex narrow phase, and its collision masks keep every pair off. Masks
+    that switch them on for one env (a copy-on-write parameter variant) are
+    accepted and stored for that env alone, and every env still equals the
+    oracle given the same masks."""
+    arr = mjcf.load_file(assets / "arm3d.xml")
+    n = 3
+    g, o = HipPhysics(arr, n), OracleSim(arr, n)
+    for s in (g, o):
+        for gi in range(int(arr["ngeom"])):
+            s.set_param("geom_contype", gi, 1, 2, [1.0])
+            s.set_param("geom_conaffinity", gi, 1, 2, [1.0])
+    assert np.array_equal(g.get_param("geom_contype", 0)[:, 0], [0, 1, 0])
+    assert np.array_equal(g.get_param("geom_conaffinity", 2)[:, 0], [0, 1, 0])
+    rng = np.random.default_rng(23)
+    v = rng.normal(size=(n, int(arr["nv"])))
+    for s in (g, o):
+        s.set("qvel", 0, n, v)
+    for t in range(150):
+        for s in (g, o):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+    assert np.array_equal(o.get("qpos"), g.get("qpos"))
 
 
 def test_contact_capacity_overflow_fails_loudly_and_freezes_the_env(assets):
