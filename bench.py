@@ -244,6 +244,10 @@ def main():
     ap.add_argument("--piles", action="store_true",
                     help="only run the tabletop pile study (8192 envs, both solvers, GPU and host oracle) "
                          "and print one JSON line per solver")
+    ap.add_argument("--shape", default="",
+                    help="launch shape 'K,specialised,schedule' (schedule: grid, ws, ws_bucket) instead of "
+                         "autotuning it on the first actions")
+    ap.add_argument("--tune-rounds", type=int, default=2, help="timed rounds over the launch shapes")
     ap.add_argument("--piles-general", action="store_true",
                     help="as --piles, on piles whose cylinders meet every body (general convex pairs)")
     args = ap.parse_args()
@@ -272,8 +276,13 @@ def main():
 
     arrays = load_model(ROOT / "pippin_amd" / "assets" / "ant.xml")
     n, sub = args.envs, args.substeps
-    nact = args.warmup + args.steps
+    TUNE_MAX = 96  # actions reserved for the autotuner's trial launches
+    tune = 0 if args.shape else TUNE_MAX
+    nact = tune + args.warmup + args.steps
     sim = HipPhysics(arrays, n)
+    if args.shape:
+        k, sp, sc = args.shape.split(",")
+        sim.set_launch_shape(int(k), int(sp), sc)
     nu = sim.nu
     q0 = initial_qpos(np.asarray(arrays["qpos0"]), n, args.seed)
     ctrl = controls(n, nu, nact, args.seed)
@@ -283,13 +292,24 @@ def main():
     stride = n * nu * 8
     log(f"bench: {n} envs x {sub} substeps, {args.warmup} warmup + {args.steps} timed actions")
 
-    for a in range(args.warmup):
+    # the autotuner tries every launch shape on live (untimed) actions; every
+    # shape gives the same trajectories, so this is warmup like any other
+    a0 = 0
+    if tune:
+        sim.autotune(rounds=args.tune_rounds)
+        while sim.autotune_pending() and a0 < tune:
+            sim.set_device("ctrl", 0, n, base + a0 * stride)
+            sim.step(0, n, sub)
+            a0 += 1
+        log(f"autotuned after {a0} actions: launch shape {sim.launch_shape()}")
+    for a in range(a0, a0 + args.warmup):
         sim.set_device("ctrl", 0, n, base + a * stride)
         sim.step(0, n, sub)
+    t_begin = a0 + args.warmup
     torch.cuda.synchronize()
     sim.kernel_timing(reset=True)
     t0 = time.perf_counter()
-    for a in range(args.warmup, nact):
+    for a in range(t_begin, t_begin + args.steps):
         sim.set_device("ctrl", 0, n, base + a * stride)
         sim.step(0, n, sub)
     torch.cuda.synchronize()
@@ -301,7 +321,8 @@ def main():
 
     # Newton iterations per step and contacts, over a few more (untimed) actions
     iters, ncon = [], []
-    for a in range(nact - min(args.iter_stats, nact), nact):
+    t_end = t_begin + args.steps
+    for a in range(t_end - min(args.iter_stats, t_end), t_end):
         sim.set_device("ctrl", 0, n, base + a * stride)
         for _ in range(sub):
             sim.step(0, n, 1)
@@ -311,7 +332,7 @@ def main():
     ncon = np.concatenate(ncon) if ncon else np.zeros(1, dtype=np.int32)
 
     cb, ops = (None, OPS_PER_ENV_STEP_DEFAULT) if args.no_cpu else cpu_baseline(
-        arrays, q0, ctrl, sub, args.warmup, args.cpu_envs)
+        arrays, q0, ctrl[:t_end], sub, t_begin, args.cpu_envs)
     # roofline of the step kernel: FP64 operations of the reference algorithm per env-step
     # (op-counting oracle on the same window) over the kernel's own time
     kernel_s = kms / 1e3 / launches
@@ -331,7 +352,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "Ant (assets/ant.xml) 16384 envs x 4 substeps per action, Newton, FP64, "
                                "ctrl U[-1,1] per action, qpos0 + N(0,0.01)",
-                   "envs": n, "substeps": sub, "actions": args.steps},
+                   "envs": n, "substeps": sub, "actions": args.steps,
+                   "launch_shape": dict(zip(("envs_per_wave", "specialised", "schedule"), sim.launch_shape())),
+                   "autotune_actions": a0},
         "newton_iters_per_step": {"mean": float(iters.mean()), "p50": float(np.percentile(iters, 50)),
                                   "p99": float(np.percentile(iters, 99))},
         "contacts_per_env": {"mean": float(ncon.mean()), "max": int(ncon.max())},

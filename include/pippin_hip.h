@@ -240,6 +240,39 @@ int pippin_hip_kernel_timing(ph_sim* s, int32_t reset, double* total_ms, int64_t
 /* Stream the library launches on (a hipStream_t). */
 void* pippin_hip_stream(ph_sim* s);
 
+/* ---- launch shapes, autotuning and graph replay of the step ----
+ * No counterpart in the reference: its CPU backend chunks envs over threads
+ * (crates/pippin/src/batch.rs:70-72) and its Metal backend dispatches one thread
+ * per env (crates/pippin-metal/src/lib.rs:174-175). Every shape computes every
+ * env with the same arithmetic, so results are identical under all of them. */
+enum {
+  PH_SCHED_GRID = 0,      /* one workgroup per K envs, dispatched by the hardware */
+  PH_SCHED_WS = 1,        /* persistent workgroups taking K envs at a time from a counter */
+  PH_SCHED_WS_BUCKET = 2  /* as WS, envs ordered by last-step Newton iterations and contacts */
+};
+typedef struct ph_launch_shape {
+  int32_t envs_per_wave; /* K: 1, 2 or 4 envs per 64-lane wavefront */
+  int32_t specialised;   /* 1: the kernel compiled for this model's sizes (when one matches) */
+  int32_t schedule;      /* PH_SCHED_* */
+  int32_t pad;
+} ph_launch_shape;
+int pippin_hip_get_launch_shape(const ph_sim* s, ph_launch_shape* out);
+/* PH_ERR_ARG for a shape the model or device cannot take (LDS, no specialised kernel). */
+int pippin_hip_set_launch_shape(ph_sim* s, const ph_launch_shape* shape);
+/* Online autotuning: the following pippin_hip_step calls (without the learned warm
+ * start) cycle through every valid shape for 1 + `rounds` rounds, timing each
+ * launch per env-step (the first round is not timed), then keep the shape with
+ * the lowest median. With `cache_path` (or NULL), a shape stored there for this
+ * model's sizes, env count and device is taken at once, and a tuned one is
+ * appended as a line "key : K specialised schedule". */
+int pippin_hip_autotune(ph_sim* s, int32_t rounds, const char* cache_path);
+/* 1 while autotuning trials remain, else 0. */
+int pippin_hip_autotune_pending(const ph_sim* s);
+/* Graph replay (default on): each plain pippin_hip_step launch sequence is
+ * captured in a hipGraph on first use and replayed while its range, step count
+ * and shape repeat (the most recent 8 are kept). */
+int pippin_hip_use_graphs(ph_sim* s, int32_t on);
+
 /* ---- learned Newton warm start ----
  * No counterpart in the reference: its warm start is the previous acceleration,
  * taken when its cost is lower than that of a0 (crates/pippin/src/newton.rs:238-245).

@@ -88,6 +88,84 @@ __global__ void __launch_bounds__(64) k_step(const DModel* __restrict__ Mp, cons
   run_env<DevXK<K>>(M, S, E, nstep, ctrl_seq, qpos_out, rel);
 }
 
+// Persistent work-stealing variant of k_step: the grid is what the device
+// holds at once, and each workgroup takes the next slot of K envs from
+// *counter until the slots run out. Slot s holds envs order[s*K .. s*K+K-1]
+// (offsets from begin; the identity without order). k_bucket writes an order
+// that puts envs of similar cost in one wavefront, costliest first, so that a
+// wavefront waits less for its slowest env and the long slots start early.
+// Every env's arithmetic is the same under any order.
+template <int K, class MT>
+__global__ void __launch_bounds__(64) k_step_ws(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                                int64_t begin, int64_t end, int64_t nstep,
+                                                const int* __restrict__ order, unsigned* counter) {
+  const MT& M = *static_cast<const MT*>(Mp);
+  const DState& S = *Sp;
+  extern __shared__ double lds[];
+  const int g = (int)threadIdx.x / DevXK<K>::G;
+  const int64_t n = end - begin;
+  const int64_t nslots = (n + K - 1) / K;
+  for (;;) {
+    unsigned slot = 0;
+    if (threadIdx.x == 0) slot = atomicAdd(counter, 1u);
+    slot = __shfl(slot, 0);
+    if ((int64_t)slot >= nslots) break;
+    const int64_t idx = (int64_t)slot * K + g;
+    if (idx < n) {
+      const int64_t rel = order ? (int64_t)order[idx] : idx;
+      Env E;
+      E.s = lds + (int64_t)g * (M.L.bytes / 8);
+      E.si = reinterpret_cast<int*>(E.s + M.L.ndouble);
+      E.e = begin + rel;
+      E.var = M.env_var[E.e];
+      run_env<DevXK<K>>(M, S, E, nstep, nullptr, nullptr, rel);
+    }
+    __syncthreads();
+  }
+}
+
+// Contact-count bucketing for k_step_ws, one workgroup: envs [begin, end)
+// keyed by their last step's Newton iterations and contacts, grouped by key,
+// costliest key first, into order (offsets from begin). Also resets the
+// work-stealing counter for the launch that follows on the same stream.
+constexpr int BUCKETS = 64;
+static __global__ void __launch_bounds__(1024) k_bucket(const DState* __restrict__ Sp, int64_t begin, int64_t end,
+                                                 int* __restrict__ order, unsigned* counter) {
+  const DState& S = *Sp;
+  __shared__ unsigned hist[BUCKETS];
+  __shared__ unsigned off[BUCKETS];
+  const int t = (int)threadIdx.x;
+  const int64_t n = end - begin;
+  if (t < BUCKETS) hist[t] = 0;
+  __syncthreads();
+  auto key = [&](int64_t e) {
+    // iterations dominate a step's cost; contacts break ties within them
+    int it = S.iters[e], nc = S.ncon[e];
+    it = it < 0 ? 0 : (it > 15 ? 15 : it);
+    nc = nc < 0 ? 0 : (nc > 3 ? 3 : nc);
+    return it * 4 + nc;
+  };
+  for (int64_t i = t; i < n; i += blockDim.x) atomicAdd(&hist[key(begin + i)], 1u);
+  __syncthreads();
+  if (t == 0) {
+    unsigned o = 0;
+    for (int b = BUCKETS - 1; b >= 0; b--) {
+      off[b] = o;
+      o += hist[b];
+    }
+    counter[0] = 0;
+  }
+  __syncthreads();
+  for (int64_t i = t; i < n; i += blockDim.x) {
+    unsigned p = atomicAdd(&off[key(begin + i)], 1u);
+    order[p] = (int)i;
+  }
+}
+
+// Resets the work-stealing counter when no bucketing pass runs before k_step_ws
+// (a kernel, not a memset, so that the launch sequence can be captured in a graph).
+static __global__ void k_reset_counter(unsigned* counter) { counter[0] = 0; }
+
 // The step with the learned warm start (mlp.hpp). Lane groups past the end
 // stay in the loop without an env, since the network needs the whole wavefront.
 template <int K, class MT>

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -26,6 +27,11 @@ using namespace phk;
 static const void* step_kernel(int K, int spec) {
   if (spec == 1 && K == 4) return (const void*)k_step<4, SpecAnt>;
   return K == 4 ? (const void*)k_step<4, DModel> : K == 2 ? (const void*)k_step<2, DModel> : (const void*)k_step<1, DModel>;
+}
+static const void* ws_kernel(int K, int spec) {
+  if (spec == 1 && K == 4) return (const void*)k_step_ws<4, SpecAnt>;
+  return K == 4 ? (const void*)k_step_ws<4, DModel>
+                : K == 2 ? (const void*)k_step_ws<2, DModel> : (const void*)k_step_ws<1, DModel>;
 }
 
 // Kinematics only (poses after set(qpos) / reset), for listed envs or a range.
@@ -158,12 +164,35 @@ struct Variants {
   int nvar = 0;
 };
 
+// A captured launch sequence of pippin_hip_step, replayed while its
+// arguments (range, step count, launch shape) repeat.
+struct StepGraph {
+  int64_t begin = 0, end = 0, nstep = 0;
+  ph_launch_shape shape{};
+  hipGraphExec_t exec = nullptr;
+};
+
 struct ph_sim {
   phh::HostModel hm;
   int64_t n = 0;
   int device = 0;
   int envs_per_wg = 1;  // environments packed into one k_step workgroup
   int spec = 0;         // 1: the model matches SpecAnt (compile-time sizes and layout)
+  int sched = PH_SCHED_GRID;  // schedule of the plain step (PH_SCHED_*)
+  int cus = 0;                // compute units of the device
+  int ws_per_cu[3][2] = {};   // resident work-stealing workgroups per CU, by K = 1, 2, 4 and kernel instance
+  std::string arch;           // device architecture name (autotune cache key)
+  // work-stealing schedule: env order (offsets from the range start) and slot counter
+  DBuf<int> d_order;
+  DBuf<uint32_t> d_counter;
+  // online autotuning: candidate shapes tried in turn on live steps
+  std::vector<ph_launch_shape> tune_cand;
+  std::vector<std::vector<double>> tune_ms;  // ms per env-step, per candidate
+  int tune_next = 0, tune_round = 0, tune_rounds = 0;
+  std::string tune_cache;
+  // graph replay of pippin_hip_step launches
+  bool use_graphs = true;
+  std::vector<StepGraph> graphs;
   // Poses are a function of qpos (PippinCpu reruns kinematics after step and
   // set(Qpos), pippin_cpu.rs:107-110, 183-185); they are recomputed for all
   // envs on the first poses() call after qpos changed.
@@ -354,12 +383,26 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   s->spec = SpecAnt::matches(M) ? 1 : 0;
   if (const char* ev = std::getenv("PIPPIN_HIP_SPEC"))
     if (std::atoi(ev) == 0) s->spec = 0;
-  {
-    const int k = s->envs_per_wg;
+  for (int k : {1, 2, 4}) {
     const int bytes = k * M.L.bytes;
-    if (bytes > 64 * 1024)
-      HIPCHK(hipFuncSetAttribute(step_kernel(k, s->spec), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    if (bytes > 64 * 1024 && bytes <= lds_max) {
+      HIPCHK(hipFuncSetAttribute(step_kernel(k, 0), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+      HIPCHK(hipFuncSetAttribute(ws_kernel(k, 0), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    }
   }
+  s->cus = prop.multiProcessorCount;
+  s->arch = prop.gcnArchName;
+  for (int ki = 0; ki < 3; ki++)
+    for (int sp = 0; sp < 2; sp++) {
+      const int k = ki == 2 ? 4 : ki + 1;
+      if ((sp == 1 && (k != 4 || !SpecAnt::matches(M))) || (int64_t)k * M.L.bytes > lds_max) continue;
+      int per_cu = 0;
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ws_kernel(k, sp), 64, (size_t)k * M.L.bytes));
+      s->ws_per_cu[ki][sp] = per_cu;
+    }
+  HIPCHK(s->d_order.alloc(std::max<int64_t>(num_envs, 1)));
+  HIPCHK(s->d_counter.alloc(1));
+  HIPCHK(s->d_counter.zero());
 
   int ii = 0, fi = 0;
   auto ui = [&](const std::vector<int32_t>& v, gp<const int>* dst) -> hipError_t { return up(s->d_i[ii++], v, dst); };
@@ -506,6 +549,7 @@ extern "C" void pippin_hip_destroy(ph_sim* s) {
   (void)hipSetDevice(s->device);
   (void)hipStreamSynchronize(s->stream);
   hipStream_t st = s->stream;
+  for (auto& g : s->graphs) (void)hipGraphExecDestroy(g.exec);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   delete s;
@@ -556,11 +600,61 @@ static int field_width(const ph_sim* s, int32_t f, double** dptr) {
   }
 }
 
+// Workgroups of the work-stealing kernel that the device holds at once
+// (occupancy taken at create, so that nothing but launches runs while a
+// step is captured in a graph).
+static unsigned ws_grid(const ph_sim* s, int K, int spec, int64_t nslots) {
+  const int per_cu = std::max(s->ws_per_cu[K == 4 ? 2 : K - 1][spec], 1);
+  return (unsigned)std::min<int64_t>((int64_t)per_cu * s->cus, nslots);
+}
+
+// The plain step (no learned warm start, no per-step controls or outputs) in
+// the given launch shape. Only kernels go on the stream, so the sequence can
+// be captured in a graph.
+static void launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t begin, int64_t end, int64_t nstep) {
+  const int K = sh.envs_per_wave, spec = sh.specialised;
+  const size_t lds = (size_t)K * s->M.L.bytes;
+  const DModel* mp = s->d_model.p;
+  const DState* sp = s->d_state.p;
+  if (sh.schedule == PH_SCHED_GRID) {
+    const unsigned grid = (unsigned)((end - begin + K - 1) / K);
+    const Real* no_ctrl = nullptr;
+    Real* no_out = nullptr;
+    void* args[] = {&mp, &sp, &begin, &end, &nstep, &no_ctrl, &no_out};
+    (void)hipLaunchKernel(step_kernel(K, spec), dim3(grid), dim3(64), args, lds, s->stream);
+    return;
+  }
+  int* order = nullptr;
+  unsigned* counter = s->d_counter.p;
+  if (sh.schedule == PH_SCHED_WS_BUCKET) {
+    order = s->d_order.p;
+    hipLaunchKernelGGL(k_bucket, dim3(1), dim3(1024), 0, s->stream, sp, begin, end, order, counter);
+  } else {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s->stream, counter);
+  }
+  const int64_t nslots = (end - begin + K - 1) / K;
+  const int* corder = order;
+  void* args[] = {&mp, &sp, &begin, &end, &nstep, &corder, &counter};
+  (void)hipLaunchKernel(ws_kernel(K, spec), dim3(ws_grid(s, K, spec, nslots)), dim3(64), args, lds, s->stream);
+}
+
+static ph_launch_shape current_shape(const ph_sim* s) {
+  ph_launch_shape sh{};
+  sh.envs_per_wave = s->envs_per_wg;
+  sh.specialised = s->spec;
+  sh.schedule = s->sched;
+  return sh;
+}
+
 // One k_step launch over [begin, end) with the env packing chosen at create.
 static void launch_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const Real* ctrl, Real* qpos_out) {
   const int K = s->envs_per_wg;
   const unsigned grid = (unsigned)((end - begin + K - 1) / K);
   const size_t lds = (size_t)K * s->M.L.bytes;
+  if (!s->M.mlp.mode && !ctrl && !qpos_out) {
+    launch_plain(s, current_shape(s), begin, end, nstep);
+    return;
+  }
   if (s->M.mlp.mode) {
     phm::launch_step(K, s->spec, grid, lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep, ctrl, qpos_out);
     if ((s->M.mlp.mode & MLP_LOG) && nstep > 0) {
@@ -581,13 +675,174 @@ static void launch_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, co
                        end, nstep, ctrl, qpos_out);
 }
 
+static bool same_shape(const ph_launch_shape& a, const ph_launch_shape& b) {
+  return a.envs_per_wave == b.envs_per_wave && a.specialised == b.specialised && a.schedule == b.schedule;
+}
+
+static bool shape_valid(const ph_sim* s, const ph_launch_shape& sh) {
+  const int K = sh.envs_per_wave;
+  if (K != 1 && K != 2 && K != 4) return false;
+  if (sh.specialised != 0 && !(sh.specialised == 1 && K == 4 && SpecAnt::matches(s->M))) return false;
+  if (sh.schedule < PH_SCHED_GRID || sh.schedule > PH_SCHED_WS_BUCKET) return false;
+  int dev_lds = 0;
+  if (hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) != hipSuccess) return false;
+  return (int64_t)K * s->M.L.bytes <= dev_lds;
+}
+
+// Key of an autotuned shape: the model's sizes and layout, env count, device.
+static std::string tune_key(const ph_sim* s) {
+  const DModel& M = s->M;
+  char buf[256];
+  snprintf(buf, sizeof buf, "v1 nq%d nv%d nu%d nb%d ng%d np%d nc%d nr%d sk%d it%d L%d n%lld %s cu%d", M.nq, M.nv, M.nu,
+           M.nbody, M.ngeom, M.npair, M.ncmax, M.nrmax, M.solver_kind, M.iterations, M.L.bytes, (long long)s->n,
+           s->arch.c_str(), s->cus);
+  std::string k = buf;
+  for (char& c : k)
+    if (c == ':') c = '_';
+  return k;
+}
+
+static void tune_finish(ph_sim* s) {
+  double best = 0.0;
+  int bi = -1;
+  for (size_t c = 0; c < s->tune_cand.size(); c++) {
+    std::vector<double> v = s->tune_ms[c];
+    if (v.empty()) continue;
+    std::sort(v.begin(), v.end());
+    const double med = v[v.size() / 2];
+    if (bi < 0 || med < best) {
+      best = med;
+      bi = (int)c;
+    }
+  }
+  if (bi >= 0) {
+    const ph_launch_shape& sh = s->tune_cand[bi];
+    s->envs_per_wg = sh.envs_per_wave;
+    s->spec = sh.specialised;
+    s->sched = sh.schedule;
+    if (!s->tune_cache.empty())
+      if (FILE* f = fopen(s->tune_cache.c_str(), "a")) {
+        fprintf(f, "%s : %d %d %d\n", tune_key(s).c_str(), sh.envs_per_wave, sh.specialised, sh.schedule);
+        fclose(f);
+      }
+  }
+  s->tune_cand.clear();
+  s->tune_ms.clear();
+  s->tune_rounds = 0;
+}
+
+extern "C" int pippin_hip_get_launch_shape(const ph_sim* s, ph_launch_shape* out) {
+  if (!s || !out) return fail(PH_ERR_ARG, "null argument");
+  *out = current_shape(s);
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_set_launch_shape(ph_sim* s, const ph_launch_shape* sh) {
+  if (!s || !sh) return fail(PH_ERR_ARG, "null argument");
+  if (!shape_valid(s, *sh)) return fail(PH_ERR_ARG, "launch shape not available for this model or device");
+  s->envs_per_wg = sh->envs_per_wave;
+  s->spec = sh->specialised;
+  s->sched = sh->schedule;
+  s->tune_cand.clear();
+  s->tune_ms.clear();
+  s->tune_rounds = 0;
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_autotune(ph_sim* s, int32_t rounds, const char* cache_path) {
+  if (!s) return fail(PH_ERR_ARG, "null sim");
+  if (rounds < 1) return fail(PH_ERR_ARG, "autotune needs at least one round");
+  s->tune_cache = cache_path ? cache_path : "";
+  if (!s->tune_cache.empty())
+    if (FILE* f = fopen(s->tune_cache.c_str(), "r")) {
+      const std::string key = tune_key(s);
+      char line[512];
+      ph_launch_shape found{};
+      bool have = false;
+      while (fgets(line, sizeof line, f)) {
+        std::string l = line;
+        const size_t colon = l.rfind(" : ");
+        if (colon == std::string::npos || l.substr(0, colon) != key) continue;
+        ph_launch_shape sh{};
+        if (sscanf(l.c_str() + colon + 3, "%d %d %d", &sh.envs_per_wave, &sh.specialised, &sh.schedule) == 3 &&
+            shape_valid(s, sh)) {
+          found = sh;
+          have = true;
+        }
+      }
+      fclose(f);
+      if (have) return pippin_hip_set_launch_shape(s, &found);
+    }
+  s->tune_cand.clear();
+  for (int spec : {1, 0})
+    for (int K : {4, 2, 1})
+      for (int sched : {PH_SCHED_GRID, PH_SCHED_WS, PH_SCHED_WS_BUCKET}) {
+        ph_launch_shape sh{};
+        sh.envs_per_wave = K;
+        sh.specialised = spec;
+        sh.schedule = sched;
+        if (shape_valid(s, sh)) s->tune_cand.push_back(sh);
+      }
+  s->tune_ms.assign(s->tune_cand.size(), {});
+  s->tune_next = 0;
+  s->tune_round = 0;
+  s->tune_rounds = rounds + 1;  // the first round loads each kernel and is not timed
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_autotune_pending(const ph_sim* s) { return s && s->tune_rounds > 0 ? 1 : 0; }
+
+extern "C" int pippin_hip_use_graphs(ph_sim* s, int32_t on) {
+  if (!s) return fail(PH_ERR_ARG, "null sim");
+  s->use_graphs = on != 0;
+  return PH_OK;
+}
+
+// The launch sequence of a plain step, from the graph cache (captured on first use).
+static int launch_plain_graph(ph_sim* s, const ph_launch_shape& sh, int64_t begin, int64_t end, int64_t nstep) {
+  for (auto& g : s->graphs)
+    if (g.begin == begin && g.end == end && g.nstep == nstep && same_shape(g.shape, sh)) {
+      HIPCHK(hipGraphLaunch(g.exec, s->stream));
+      return PH_OK;
+    }
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  launch_plain(s, sh, begin, end, nstep);
+  hipError_t launch_err = hipGetLastError();
+  hipError_t cap_err = hipStreamEndCapture(s->stream, &graph);
+  HIPCHK(launch_err);
+  HIPCHK(cap_err);
+  StepGraph g;
+  g.begin = begin;
+  g.end = end;
+  g.nstep = nstep;
+  g.shape = sh;
+  hipError_t inst = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPCHK(inst);
+  if (s->graphs.size() >= 8) {  // keep the most recent few
+    (void)hipGraphExecDestroy(s->graphs.front().exec);
+    s->graphs.erase(s->graphs.begin());
+  }
+  s->graphs.push_back(g);
+  HIPCHK(hipGraphLaunch(g.exec, s->stream));
+  return PH_OK;
+}
+
 extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep) {
   if (int rc = check_range(s, begin, end)) return rc;
   if (nstep < 0) return fail(PH_ERR_ARG, "negative step count");
   if (end == begin) return PH_OK;
   HIPCHK(hipSetDevice(s->device));
+  const bool plain = !s->M.mlp.mode;
+  const bool trial = plain && s->tune_rounds > 0 && nstep > 0;
+  const ph_launch_shape sh = trial ? s->tune_cand[s->tune_next] : current_shape(s);
   HIPCHK(hipEventRecord(s->ev0, s->stream));
-  launch_step(s, begin, end, nstep, nullptr, nullptr);
+  if (!plain) launch_step(s, begin, end, nstep, nullptr, nullptr);
+  else if (s->use_graphs && !trial) {
+    if (int rc = launch_plain_graph(s, sh, begin, end, nstep)) return rc;
+  } else
+    launch_plain(s, sh, begin, end, nstep);
   s->poses_stale = true;
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, s->stream));
@@ -596,6 +851,13 @@ extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t ns
   HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   s->kernel_ms += ms;
   s->kernel_launches++;
+  if (trial) {
+    if (s->tune_round > 0) s->tune_ms[s->tune_next].push_back(ms / ((double)(end - begin) * (double)nstep));
+    if (++s->tune_next == (int)s->tune_cand.size()) {
+      s->tune_next = 0;
+      if (++s->tune_round == s->tune_rounds) tune_finish(s);
+    }
+  }
   return status_of(s, begin, end);
 }
 

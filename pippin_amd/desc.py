@@ -86,6 +86,11 @@ class ph_contact(C.Structure):
                 ("feature", C.c_int32), ("pad", C.c_int32)]
 
 
+class ph_launch_shape(C.Structure):
+    _fields_ = [("envs_per_wave", C.c_int32), ("specialised", C.c_int32), ("schedule", C.c_int32),
+                ("pad", C.c_int32)]
+
+
 class ph_options(C.Structure):
     _fields_ = [("device", C.c_int32), ("solver_override", C.c_int32), ("max_contacts", C.c_int32),
                 ("reserved", C.c_int32 * 5)]

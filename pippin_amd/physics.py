@@ -15,7 +15,7 @@ from pathlib import Path
 import numpy as np
 
 from .desc import (APPEARANCE_DTYPE, CONTACT_DTYPE, POSE_DTYPE, Desc, arrays_from_desc, ph_dims,
-                   ph_model_desc, ph_options)
+                   ph_launch_shape, ph_model_desc, ph_options)
 
 LIB_PATH = Path(__file__).resolve().parent / "libpippin_hip.so"
 FIELD = {"qpos": 0, "qvel": 1, "ctrl": 2}
@@ -82,6 +82,11 @@ def lib():
             "pippin_hip_selftest_math": ([vp, i64, vp, i32], C.c_int),
             "pippin_hip_set_device": ([vp, i32, i64, i64, vp], C.c_int),
             "pippin_hip_kernel_timing": ([vp, i32, C.POINTER(C.c_double), C.POINTER(i64)], C.c_int),
+            "pippin_hip_get_launch_shape": ([vp, C.POINTER(ph_launch_shape)], C.c_int),
+            "pippin_hip_set_launch_shape": ([vp, C.POINTER(ph_launch_shape)], C.c_int),
+            "pippin_hip_autotune": ([vp, i32, C.c_char_p], C.c_int),
+            "pippin_hip_autotune_pending": ([vp], C.c_int),
+            "pippin_hip_use_graphs": ([vp, i32], C.c_int),
             "pippin_hip_mlp_init": ([vp, i32, C.c_uint64, i64, i64], C.c_int),
             "pippin_hip_mlp_mode": ([vp, i32], C.c_int),
             "pippin_hip_mlp_collect": ([vp, C.POINTER(i64)], C.c_int),
@@ -292,6 +297,32 @@ class HipPhysics:
     def set_device(self, field, begin, end, dev_ptr: int):
         """Copy a field from device memory, ordered before the next step."""
         self._check(self.L.pippin_hip_set_device(self.h, FIELD[field], begin, end, C.c_void_p(dev_ptr)))
+
+    SCHED = {"grid": 0, "ws": 1, "ws_bucket": 2}
+
+    def launch_shape(self):
+        """(envs per wavefront, specialised kernel, schedule name) of the plain step."""
+        sh = ph_launch_shape()
+        self._check(self.L.pippin_hip_get_launch_shape(self.h, C.byref(sh)))
+        names = {v: k for k, v in self.SCHED.items()}
+        return int(sh.envs_per_wave), int(sh.specialised), names[int(sh.schedule)]
+
+    def set_launch_shape(self, envs_per_wave: int, specialised: int, schedule: str):
+        sh = ph_launch_shape(int(envs_per_wave), int(specialised), self.SCHED[schedule], 0)
+        if self.L.pippin_hip_set_launch_shape(self.h, C.byref(sh)) != 0:
+            raise HipError(_err())
+
+    def autotune(self, rounds: int = 3, cache_path: str | None = None):
+        """Try every launch shape on the next plain steps, then keep the fastest
+        (results are identical under every shape)."""
+        path = None if cache_path is None else str(cache_path).encode()
+        self._check(self.L.pippin_hip_autotune(self.h, int(rounds), path))
+
+    def autotune_pending(self) -> bool:
+        return bool(self.L.pippin_hip_autotune_pending(self.h))
+
+    def use_graphs(self, on: bool):
+        self._check(self.L.pippin_hip_use_graphs(self.h, int(bool(on))))
 
     def kernel_timing(self, reset=False):
         ms, n = C.c_double(0), C.c_int64(0)

@@ -1,0 +1,145 @@
+"""Launch shapes, autotuning and graph replay of the step (needs a GPU).
+
+Every shape (envs per wavefront, generic or model-specialised kernel, grid,
+work-stealing or bucketed work-stealing schedule) computes every env with the
+same arithmetic, so trajectories must be bit-identical under all of them, also
+while the autotuner switches shapes between launches and when launches are
+replayed from captured graphs.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mjcf
+from pippin_amd.physics import HipError, HipPhysics
+from pippin_amd.workloads import random_pile_xml
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(k, sp, sc) for sp in (1, 0) for k in (4, 2, 1) for sc in ("grid", "ws", "ws_bucket")]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _ant(assets, n, seed):
+    arr = mjcf.load_file(assets / "ant.xml")
+    rng = np.random.default_rng(seed)
+    g = HipPhysics(arr, n)
+    q = g.get("qpos") + rng.normal(size=(n, int(arr["nq"]))) * 0.01
+    q[:, 3:7] /= np.linalg.norm(q[:, 3:7], axis=1, keepdims=True)
+    return arr, q
+
+
+def _run(arr, q, n, shape, calls, seed, graphs=True, ranges=None):
+    g = HipPhysics(arr, n)
+    if shape is not None:
+        g.set_launch_shape(*shape)
+    g.use_graphs(graphs)
+    g.set("qpos", 0, n, q)
+    rng = np.random.default_rng(seed)
+    nu = int(arr["nu"])
+    out = []
+    for t in range(calls):
+        if nu:
+            g.set("ctrl", 0, n, rng.uniform(-1, 1, size=(n, nu)))
+        b, e = ranges[t % len(ranges)] if ranges else (0, n)
+        g.step(b, e, 4)
+        out.append((g.get("qpos").copy(), g.ncon().copy()))
+    g.close()
+    return out
+
+
+def _same(a, b):
+    return all(np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) for x, y in zip(a, b))
+
+
+def test_every_shape_matches_on_ant(assets):
+    n = 13  # not a multiple of 2 or 4: ragged last wavefront
+    arr, q = _ant(assets, n, 3)
+    ranges = [(0, n), (0, n), (3, n), (0, 9)]
+    ref = _run(arr, q, n, (1, 0, "grid"), 24, 5, graphs=False, ranges=ranges)
+    assert sum(int(c.sum()) for _, c in ref) > 0
+    tried = 0
+    for shape in SHAPES:
+        try:
+            got = _run(arr, q, n, shape, 24, 5, ranges=ranges)
+        except HipError:
+            continue  # not available for this model (e.g. no specialised kernel for K < 4)
+        say("ant shape", shape)
+        assert _same(ref, got), shape
+        tried += 1
+    assert tried >= 10
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_every_shape_matches_on_a_pile(solver):
+    arr = mjcf.load_string(random_pile_xml(29))
+    arr["solver_kind"] = solver
+    n = 11
+    rng = np.random.default_rng(29)
+    g = HipPhysics(arr, n)
+    q = g.get("qpos")
+    g.close()
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    ref = _run(arr, q, n, None, 30, 7, graphs=False)
+    tried = 0
+    for shape in SHAPES:
+        try:
+            got = _run(arr, q, n, shape, 30, 7)
+        except HipError:
+            continue
+        say("pile shape", solver, shape)
+        assert _same(ref, got), shape
+        tried += 1
+    assert tried >= 3
+
+
+def test_autotune_keeps_results_and_caches_its_choice(assets, tmp_path):
+    n = 64
+    arr, q = _ant(assets, n, 11)
+    cache = tmp_path / "shapes.txt"
+    plain = _run(arr, q, n, None, 1, 13)  # warm the kernels
+    g, ref = HipPhysics(arr, n), HipPhysics(arr, n)
+    for s in (g, ref):
+        s.set("qpos", 0, n, q)
+    g.autotune(rounds=1, cache_path=cache)
+    rng = np.random.default_rng(17)
+    calls = 0
+    while g.autotune_pending():
+        c = rng.uniform(-1, 1, size=(n, int(arr["nu"])))
+        for s in (g, ref):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 2)
+        assert np.array_equal(g.get("qpos"), ref.get("qpos")), calls
+        calls += 1
+        assert calls < 200
+    k, sp, sc = g.launch_shape()
+    say("autotuned", (k, sp, sc), "after", calls, "launches")
+    assert calls >= 10 and k in (1, 2, 4) and sc in ("grid", "ws", "ws_bucket")
+    assert cache.read_text().strip().endswith(f"{k} {sp} {HipPhysics.SCHED[sc]}")
+    h = HipPhysics(arr, n)
+    h.autotune(rounds=1, cache_path=cache)
+    assert not h.autotune_pending() and h.launch_shape() == (k, sp, sc)
+    for s in (g, ref, h):
+        s.close()
+    del plain
+
+
+@pytest.mark.parametrize("schedule", ["grid", "ws_bucket"])
+def test_graph_replays_match_eager_launches(assets, schedule):
+    """Replays of captured step graphs, with the program's own eager work
+    (controls copied in, state read back) between them, against the same
+    launches made eagerly."""
+    n = 16
+    arr, q = _ant(assets, n, 21)
+    ranges = [(0, n), (4, n), (0, n), (0, 8)]
+    eager = _run(arr, q, n, (4, 0, schedule), 24, 23, graphs=False, ranges=ranges)
+    graph = _run(arr, q, n, (4, 0, schedule), 24, 23, graphs=True, ranges=ranges)
+    for t, (a, b) in enumerate(zip(eager, graph)):
+        assert np.array_equal(a[0], b[0]), t
+        assert np.array_equal(a[1], b[1]), t
