@@ -37,8 +37,8 @@ OPS_PER_ENV_STEP_DEFAULT = 25000.0
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_step launch of the default workload (16384 envs x 4 substeps),
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a known-size copy
-# (profiles/k_step_ant16k.md); other workloads report null.
-TRAFFIC_BYTES_PER_LAUNCH = 33.4e6
+# (profiles/k_step_ws_ant16k.md); other workloads report null.
+TRAFFIC_BYTES_PER_LAUNCH = 40.2e6
 
 
 def log(*a):
@@ -248,6 +248,7 @@ def main():
                     help="launch shape 'K,specialised,schedule' (schedule: grid, ws, ws_bucket) instead of "
                          "autotuning it on the first actions")
     ap.add_argument("--tune-rounds", type=int, default=2, help="timed rounds over the launch shapes")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every step eagerly instead of graph replay")
     ap.add_argument("--piles-general", action="store_true",
                     help="as --piles, on piles whose cylinders meet every body (general convex pairs)")
     args = ap.parse_args()
@@ -283,6 +284,7 @@ def main():
     if args.shape:
         k, sp, sc = args.shape.split(",")
         sim.set_launch_shape(int(k), int(sp), sc)
+    sim.use_graphs(not args.no_graphs)
     nu = sim.nu
     q0 = initial_qpos(np.asarray(arrays["qpos0"]), n, args.seed)
     ctrl = controls(n, nu, nact, args.seed)
