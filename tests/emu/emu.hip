@@ -10,6 +10,9 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <algorithm>
+#include <cstdlib>
+#include <limits>
 #include <vector>
 
 #include "../../include/pippin_hip.h"
@@ -213,6 +216,19 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   s->islf.assign(std::max<int64_t>(nn * (int64_t)M.isl_fstride, 1), 0.0);
   s->gmj.assign(M.solver_kind == 1 ? nn * (int64_t)M.nrmax * d.nv : 1, 0.0);
   s->isli.assign(std::max<int64_t>(nn * (int64_t)M.isl_istride, 1), 0);
+  // EMU_POISON_SCRATCH=1 fills the buffers the device library leaves
+  // uninitialised at create (contacts, staging, row spill, island scratch)
+  // with NaN / -1, so that a read before write differs from the oracle
+  if (const char* ps = std::getenv("EMU_POISON_SCRATCH")) {
+    // "1": NaN and -1; any other number x: doubles x, ints (int)x
+    const double x = std::atof(ps);
+    const double dv = std::string(ps) == "1" ? std::numeric_limits<double>::quiet_NaN() : x;
+    const int iv = std::string(ps) == "1" ? -1 : (int)x;
+    for (auto* v : {&s->gxpos, &s->gxmat, &s->xpos, &s->xmat, &s->con, &s->pcon, &s->plambda, &s->stage, &s->gjac,
+                    &s->grow, &s->islf, &s->gmj})
+      std::fill(v->begin(), v->end(), dv);
+    for (auto* v : {&s->cgeom, &s->pcgeom, &s->isli}) std::fill(v->begin(), v->end(), iv);
+  }
   s->ncon.assign(nn, 0);
   s->iters.assign(nn, 0);
   s->nrows.assign(nn, 0);
@@ -247,6 +263,7 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   S.gmj = s->gmj.data();
   S.isl_i = s->isli.data();
   s->lds.assign(M.L.bytes / sizeof(double) + 16, 0.0);
+  if (std::getenv("EMU_POISON_LDS")) std::fill(s->lds.begin(), s->lds.end(), std::numeric_limits<double>::quiet_NaN());
   // set_const for the base variant
   Env E;
   E.s = s->lds.data();
@@ -270,9 +287,13 @@ int emu_step(emu_sim* s, int64_t begin, int64_t end, int64_t nstep) {
   Env E;
   E.s = s->lds.data();
   E.si = reinterpret_cast<int*>(E.s + s->M.L.ndouble);
+  // EMU_POISON_LDS=1 fills the working set with NaN before each env, so that a
+  // read of LDS the step has not written shows up as a difference from the oracle
+  const bool poison = std::getenv("EMU_POISON_LDS") != nullptr;
   for (int64_t e = begin; e < end; e++) {
     E.e = e;
     E.var = s->env_var[e];
+    if (poison) std::fill(s->lds.begin(), s->lds.end(), std::numeric_limits<double>::quiet_NaN());
     if (s->M.mlp.mode)
       run_env_mlp<HostXM>(s->M, s->S, E, nstep, nullptr, nullptr, e - begin, true);
     else
