@@ -242,6 +242,62 @@ def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = Fal
     return out
 
 
+def arm_study(n: int, steps: int, seed: int, cpu_envs: int):
+    """The arm configuration (pippin_amd/workloads.py arm_gripper_xml: 7-DoF
+    arm, tendon-coupled two-finger gripper, cube on a table, implicitfast),
+    cube size and friction varied over 16 env groups, joint targets on smooth
+    random waypoints every step. Batch::rollout on the device (per-step
+    controls resident in HBM, 50 steps per launch) against the oracle's
+    rollout of a sample of the same envs on the host cores."""
+    import torch
+    from oracle.pyoracle import OracleSim
+    from pippin_amd.physics import HipPhysics, load_model
+    from pippin_amd.workloads import arm_gripper_xml, arm_targets
+
+    threads = len(os.sched_getaffinity(0))
+    arrays = load_model(arm_gripper_xml(seed), is_xml=True)
+    nu, cube = int(arrays["nu"]), int(arrays["ngeom"]) - 1
+    warm, chunk = 50, 50
+    tg = arm_targets(seed, n, warm + steps, nu)  # (steps, envs, nu)
+    sim = HipPhysics(arrays, n)
+    ref = OracleSim(arrays, min(cpu_envs, n))
+    rng = np.random.default_rng(seed)
+    g = max(n // 16, 1)
+    for b in range(0, n, g):
+        h, f = rng.uniform(0.02, 0.03), rng.uniform(0.6, 1.0)
+        e = min(b + g, n)
+        sim.set_param("geom_size", cube, b, e, [[h, h, h]] * (e - b))
+        sim.set_param("geom_friction", cube, b, e, [[f]] * (e - b))
+        if b < ref.n:
+            re = min(e, ref.n)
+            ref.set_param("geom_size", cube, b, re, [[h, h, h]] * (re - b))
+            ref.set_param("geom_friction", cube, b, re, [[f]] * (re - b))
+    ctrl = np.ascontiguousarray(np.transpose(tg, (1, 0, 2)))  # envs x steps x nu
+    dctrl = torch.from_numpy(ctrl).to("cuda")
+    for c0 in range(0, warm, chunk):  # warmup (untimed)
+        part = dctrl[:, c0:c0 + chunk].contiguous()
+        sim.rollout_device(part.data_ptr(), None, part.shape[1])
+    parts = [dctrl[:, c0:c0 + chunk].contiguous() for c0 in range(warm, warm + steps, chunk)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for part in parts:
+        sim.rollout_device(part.data_ptr(), None, part.shape[1])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ncon = float(sim.ncon().mean())
+    iters = float(sim.solver_iterations().mean())
+    sim.close()
+    m = ref.n
+    ref.rollout(ctrl[:m, :warm], threads=threads, want_qpos=False)
+    c0 = time.perf_counter()
+    ref.rollout(ctrl[:m, warm:], threads=threads, want_qpos=False)
+    cdt = time.perf_counter() - c0
+    return {"workload": f"7-DoF arm + tendon gripper + cube (seed {seed}), implicitfast, Newton, rollout with "
+                        f"per-step targets", "envs": n, "steps": steps,
+            "gpu_env_steps_per_s": n * steps / dt, "cpu_env_steps_per_s": m * steps / cdt, "cpu_envs": m,
+            "cpu_threads": threads, "contacts_mean": ncon, "newton_iters_mean": iters}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,9 +325,14 @@ def main():
                          "autotuning it on the first actions")
     ap.add_argument("--tune-rounds", type=int, default=2, help="timed rounds over the launch shapes")
     ap.add_argument("--no-graphs", action="store_true", help="launch every step eagerly instead of graph replay")
+    ap.add_argument("--arm", action="store_true",
+                    help="only run the arm study (4096 envs, rollout with per-step targets, GPU and host oracle)")
     ap.add_argument("--piles-general", action="store_true",
                     help="as --piles, on piles whose cylinders meet every body (general convex pairs)")
     args = ap.parse_args()
+    if args.arm:
+        print(json.dumps(arm_study(4096, 200, 0, 512)), flush=True)
+        return
     if args.piles or args.piles_general:
         seeds = (13, 18) if args.piles_general else (11, 29)
         for seed in seeds:

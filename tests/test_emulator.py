@@ -373,3 +373,22 @@ def test_random_piles_with_general_pairs_exact(seed, solver):
         assert np.array_equal(o.ncon(), e.ncon()), t
         assert np.array_equal(o.get("qpos"), e.get("qpos")), t
     assert o.ncon().sum() > 0
+
+
+def test_arm_gripper_exact():
+    """The 7-DoF arm with tendon-coupled gripper (pippin_amd/workloads.py
+    arm_gripper_xml): implicitfast, position actuators, fixed tendon, joint
+    equality, limits and hand/cube/table contacts, on smooth random targets."""
+    from pippin_amd.workloads import arm_gripper_xml, arm_targets
+
+    arr = mjcf.load_string(arm_gripper_xml(3))
+    n = 4
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    tg = arm_targets(3, n, 150, int(arr["nu"]))
+    for t in range(150):
+        for s in (o, e):
+            s.set("ctrl", 0, n, tg[t])
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert o.ncon().sum() > 0

@@ -390,3 +390,31 @@ def test_random_piles_with_general_pairs_match(seed, solver):
         assert np.array_equal(o.ncon(), g.ncon()), t
         assert np.array_equal(o.get("qpos"), g.get("qpos")), t
     assert o.ncon().sum() > 0
+
+
+def test_arm_gripper_with_per_env_cubes_match():
+    """The 7-DoF arm with tendon-coupled gripper, cube size and friction
+    varied per env group (copy-on-write parameter variants), 200 steps on
+    smooth random targets."""
+    from pippin_amd.workloads import arm_gripper_xml, arm_targets
+
+    arr = mjcf.load_string(arm_gripper_xml(5))
+    n = 24
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    cube = int(arr["ngeom"]) - 1
+    rng = np.random.default_rng(5)
+    for b in range(0, n, 6):
+        h = rng.uniform(0.02, 0.03)
+        f = rng.uniform(0.6, 1.0)
+        for s in (o, g):
+            s.set_param("geom_size", cube, b, b + 6, [[h, h, h]] * 6)
+            s.set_param("geom_friction", cube, b, b + 6, [[f]] * 6)
+    tg = arm_targets(5, n, 200, int(arr["nu"]))
+    say("arm gripper", n)
+    for t in range(200):
+        for s in (o, g):
+            s.set("ctrl", 0, n, tg[t])
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert o.ncon().sum() > 0
