@@ -257,8 +257,8 @@ def arm_study(n: int, steps: int, seed: int, cpu_envs: int):
     threads = len(os.sched_getaffinity(0))
     arrays = load_model(arm_gripper_xml(seed), is_xml=True)
     nu, cube = int(arrays["nu"]), int(arrays["ngeom"]) - 1
-    warm, chunk = 50, 50
-    tg = arm_targets(seed, n, warm + steps, nu)  # (steps, envs, nu)
+    warm, chunk, tune_steps = 50, 50, 400
+    tg = arm_targets(seed, n, tune_steps + warm + steps, nu)  # (steps, envs, nu)
     sim = HipPhysics(arrays, n)
     ref = OracleSim(arrays, min(cpu_envs, n))
     rng = np.random.default_rng(seed)
@@ -274,10 +274,19 @@ def arm_study(n: int, steps: int, seed: int, cpu_envs: int):
             ref.set_param("geom_friction", cube, b, re, [[f]] * (re - b))
     ctrl = np.ascontiguousarray(np.transpose(tg, (1, 0, 2)))  # envs x steps x nu
     dctrl = torch.from_numpy(ctrl).to("cuda")
-    for c0 in range(0, warm, chunk):  # warmup (untimed)
+    # autotune the launch shape on 10-step rollouts (untimed; results do not depend on it)
+    sim.autotune(rounds=1)
+    c0 = 0
+    while sim.autotune_pending() and c0 + 10 <= tune_steps:
+        part = dctrl[:, c0:c0 + 10].contiguous()
+        sim.rollout_device(part.data_ptr(), None, 10)
+        c0 += 10
+    tuned = c0
+    for c0 in range(tuned, tuned + warm, chunk):  # warmup (untimed)
         part = dctrl[:, c0:c0 + chunk].contiguous()
         sim.rollout_device(part.data_ptr(), None, part.shape[1])
-    parts = [dctrl[:, c0:c0 + chunk].contiguous() for c0 in range(warm, warm + steps, chunk)]
+    t_begin = tuned + warm
+    parts = [dctrl[:, c0:c0 + chunk].contiguous() for c0 in range(t_begin, t_begin + steps, chunk)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for part in parts:
@@ -286,16 +295,18 @@ def arm_study(n: int, steps: int, seed: int, cpu_envs: int):
     dt = time.perf_counter() - t0
     ncon = float(sim.ncon().mean())
     iters = float(sim.solver_iterations().mean())
+    shape = sim.launch_shape()
     sim.close()
     m = ref.n
-    ref.rollout(ctrl[:m, :warm], threads=threads, want_qpos=False)
+    ref.rollout(ctrl[:m, :t_begin], threads=threads, want_qpos=False)
     c0 = time.perf_counter()
-    ref.rollout(ctrl[:m, warm:], threads=threads, want_qpos=False)
+    ref.rollout(ctrl[:m, t_begin:t_begin + steps], threads=threads, want_qpos=False)
     cdt = time.perf_counter() - c0
     return {"workload": f"7-DoF arm + tendon gripper + cube (seed {seed}), implicitfast, Newton, rollout with "
                         f"per-step targets", "envs": n, "steps": steps,
             "gpu_env_steps_per_s": n * steps / dt, "cpu_env_steps_per_s": m * steps / cdt, "cpu_envs": m,
-            "cpu_threads": threads, "contacts_mean": ncon, "newton_iters_mean": iters}
+            "cpu_threads": threads, "contacts_mean": ncon, "newton_iters_mean": iters, "launch_shape": shape,
+            "autotune_steps": tuned}
 
 
 def main():

@@ -259,8 +259,9 @@ typedef struct ph_launch_shape {
 int pippin_hip_get_launch_shape(const ph_sim* s, ph_launch_shape* out);
 /* PH_ERR_ARG for a shape the model or device cannot take (LDS, no specialised kernel). */
 int pippin_hip_set_launch_shape(ph_sim* s, const ph_launch_shape* shape);
-/* Online autotuning: the following pippin_hip_step calls (without the learned warm
- * start) cycle through every valid shape for 1 + `rounds` rounds, timing each
+/* Online autotuning: the following pippin_hip_step and pippin_hip_rollout_device
+ * calls (without the learned warm start) cycle through every valid shape for
+ * 1 + `rounds` rounds, timing each
  * launch per env-step (the first round is not timed), then keep the shape with
  * the lowest median. With `cache_path` (or NULL), a shape stored there for this
  * model's sizes, env count and device is taken at once, and a tuned one is

@@ -94,11 +94,12 @@ __global__ void __launch_bounds__(64) k_step(const DModel* __restrict__ Mp, cons
 // (offsets from begin; the identity without order). k_bucket writes an order
 // that puts envs of similar cost in one wavefront, costliest first, so that a
 // wavefront waits less for its slowest env and the long slots start early.
-// Every env's arithmetic is the same under any order.
+// Every env's arithmetic is the same under any order. ctrl_seq / qpos_out as
+// for k_step (rows indexed by the env's offset from begin).
 template <int K, class MT>
 __global__ void __launch_bounds__(64) k_step_ws(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
-                                                int64_t begin, int64_t end, int64_t nstep,
-                                                const int* __restrict__ order, unsigned* counter) {
+                                                int64_t begin, int64_t end, int64_t nstep, const Real* ctrl_seq,
+                                                Real* qpos_out, const int* __restrict__ order, unsigned* counter) {
   const MT& M = *static_cast<const MT*>(Mp);
   const DState& S = *Sp;
   extern __shared__ double lds[];
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(64) k_step_ws(const DModel* __restrict__ Mp, c
       E.si = reinterpret_cast<int*>(E.s + M.L.ndouble);
       E.e = begin + rel;
       E.var = M.env_var[E.e];
-      run_env<DevXK<K>>(M, S, E, nstep, nullptr, nullptr, rel);
+      run_env<DevXK<K>>(M, S, E, nstep, ctrl_seq, qpos_out, rel);
     }
     __syncthreads();
   }

@@ -608,19 +608,18 @@ static unsigned ws_grid(const ph_sim* s, int K, int spec, int64_t nslots) {
   return (unsigned)std::min<int64_t>((int64_t)per_cu * s->cus, nslots);
 }
 
-// The plain step (no learned warm start, no per-step controls or outputs) in
-// the given launch shape. Only kernels go on the stream, so the sequence can
-// be captured in a graph.
-static void launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t begin, int64_t end, int64_t nstep) {
+// The step without the learned warm start in the given launch shape, with
+// optional per-step controls and qpos outputs (Batch::rollout). Only kernels
+// go on the stream, so the sequence can be captured in a graph.
+static void launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t begin, int64_t end, int64_t nstep,
+                         const Real* ctrl = nullptr, Real* qpos_out = nullptr) {
   const int K = sh.envs_per_wave, spec = sh.specialised;
   const size_t lds = (size_t)K * s->M.L.bytes;
   const DModel* mp = s->d_model.p;
   const DState* sp = s->d_state.p;
   if (sh.schedule == PH_SCHED_GRID) {
     const unsigned grid = (unsigned)((end - begin + K - 1) / K);
-    const Real* no_ctrl = nullptr;
-    Real* no_out = nullptr;
-    void* args[] = {&mp, &sp, &begin, &end, &nstep, &no_ctrl, &no_out};
+    void* args[] = {&mp, &sp, &begin, &end, &nstep, &ctrl, &qpos_out};
     (void)hipLaunchKernel(step_kernel(K, spec), dim3(grid), dim3(64), args, lds, s->stream);
     return;
   }
@@ -634,7 +633,7 @@ static void launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t begin, in
   }
   const int64_t nslots = (end - begin + K - 1) / K;
   const int* corder = order;
-  void* args[] = {&mp, &sp, &begin, &end, &nstep, &corder, &counter};
+  void* args[] = {&mp, &sp, &begin, &end, &nstep, &ctrl, &qpos_out, &corder, &counter};
   (void)hipLaunchKernel(ws_kernel(K, spec), dim3(ws_grid(s, K, spec, nslots)), dim3(64), args, lds, s->stream);
 }
 
@@ -651,8 +650,8 @@ static void launch_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, co
   const int K = s->envs_per_wg;
   const unsigned grid = (unsigned)((end - begin + K - 1) / K);
   const size_t lds = (size_t)K * s->M.L.bytes;
-  if (!s->M.mlp.mode && !ctrl && !qpos_out) {
-    launch_plain(s, current_shape(s), begin, end, nstep);
+  if (!s->M.mlp.mode) {
+    launch_plain(s, current_shape(s), begin, end, nstep, ctrl, qpos_out);
     return;
   }
   if (s->M.mlp.mode) {
@@ -829,20 +828,20 @@ static int launch_plain_graph(ph_sim* s, const ph_launch_shape& sh, int64_t begi
   return PH_OK;
 }
 
-extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep) {
-  if (int rc = check_range(s, begin, end)) return rc;
-  if (nstep < 0) return fail(PH_ERR_ARG, "negative step count");
-  if (end == begin) return PH_OK;
+// One timed launch of the step over [begin, end) for pippin_hip_step and
+// pippin_hip_rollout_device: an autotuning trial shape while trials remain,
+// else the current shape (from the graph cache for a plain step).
+static int timed_launch(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const Real* ctrl, Real* qpos_out) {
   HIPCHK(hipSetDevice(s->device));
   const bool plain = !s->M.mlp.mode;
   const bool trial = plain && s->tune_rounds > 0 && nstep > 0;
   const ph_launch_shape sh = trial ? s->tune_cand[s->tune_next] : current_shape(s);
   HIPCHK(hipEventRecord(s->ev0, s->stream));
-  if (!plain) launch_step(s, begin, end, nstep, nullptr, nullptr);
-  else if (s->use_graphs && !trial) {
+  if (!plain) launch_step(s, begin, end, nstep, ctrl, qpos_out);
+  else if (s->use_graphs && !trial && !ctrl && !qpos_out) {
     if (int rc = launch_plain_graph(s, sh, begin, end, nstep)) return rc;
   } else
-    launch_plain(s, sh, begin, end, nstep);
+    launch_plain(s, sh, begin, end, nstep, ctrl, qpos_out);
   s->poses_stale = true;
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, s->stream));
@@ -859,6 +858,13 @@ extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t ns
     }
   }
   return status_of(s, begin, end);
+}
+
+extern "C" int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  if (nstep < 0) return fail(PH_ERR_ARG, "negative step count");
+  if (end == begin) return PH_OK;
+  return timed_launch(s, begin, end, nstep, nullptr, nullptr);
 }
 
 extern "C" int pippin_hip_kernel_timing(ph_sim* s, int32_t reset, double* total_ms, int64_t* launches) {
@@ -887,12 +893,7 @@ extern "C" int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, 
   if (int rc = check_range(s, begin, end)) return rc;
   if (nstep < 0 || !ctrl) return fail(PH_ERR_ARG, "bad rollout arguments");
   if (end == begin) return PH_OK;
-  HIPCHK(hipSetDevice(s->device));
-  launch_step(s, begin, end, nstep, ctrl, qpos_out);
-  s->poses_stale = true;
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s->stream));
-  return status_of(s, begin, end);
+  return timed_launch(s, begin, end, nstep, ctrl, qpos_out);
 }
 
 extern "C" int pippin_hip_state_device(ph_sim* s, int32_t field, double** ptr) {

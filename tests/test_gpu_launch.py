@@ -143,3 +143,41 @@ def test_graph_replays_match_eager_launches(assets, schedule):
     for t, (a, b) in enumerate(zip(eager, graph)):
         assert np.array_equal(a[0], b[0]), t
         assert np.array_equal(a[1], b[1]), t
+
+
+def test_rollout_matches_under_every_shape_and_while_autotuning():
+    """Batch::rollout with per-step controls and qpos outputs in device
+    memory, under every shape and during autotuning trials."""
+    import torch
+
+    arr = mjcf.load_string(random_pile_xml(29))
+    n, T = 11, 12
+    rng = np.random.default_rng(31)
+    q0 = None
+    outs = []
+    shapes = [None] + [s for s in SHAPES if s[1] == 0] + ["autotune"]
+    for shape in shapes:
+        g = HipPhysics(arr, n)
+        if q0 is None:
+            q0 = g.get("qpos")
+            for b in range(int(arr["nq"]) // 7):
+                q0[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+        g.set("qpos", 0, n, q0)
+        if shape == "autotune":
+            g.autotune(rounds=1)
+        elif shape is not None:
+            g.set_launch_shape(*shape)
+        nu = max(int(arr["nu"]), 1)
+        ctrl = torch.zeros((n, T, nu), dtype=torch.float64, device="cuda")
+        qo = torch.zeros((n, T, int(arr["nq"])), dtype=torch.float64, device="cuda")
+        res = []
+        for call in range(12):
+            g.rollout_device(ctrl.data_ptr(), qo.data_ptr(), T)
+            torch.cuda.synchronize()
+            res.append(qo.cpu().numpy().copy())
+        outs.append(res)
+        say("rollout shape", shape)
+        g.close()
+    for shape, res in zip(shapes[1:], outs[1:]):
+        for a, b in zip(outs[0], res):
+            assert np.array_equal(a, b), shape
