@@ -141,6 +141,7 @@ struct DState {
   gp<Real> qpos, qvel, ctrl, qacc, applied, time;
   gp<Real> geom_xpos, geom_xmat, xpos, xmat;  // poses after the last step
   gp<int> ncon, iters, nrows, status;
+  gp<int> cost;  // Newton iterations summed over the env's last launch (bucketing key)
   gp<Real> con;       // n x ncmax x CW
   gp<int> cgeom;      // n x ncmax x 3 (g1, g2, feature)
   gp<Real> pcon;      // PGS history: previous contacts

@@ -1906,10 +1906,12 @@ PHD void run_env(const MT& M, const DState& S, Env& E, int64_t nstep, const Real
   load_state<X>(M, S, E);
   int* isc = E.si + M.L.i_iscal;
   const int64_t e = E.e;
+  int cost = 0;
   for (int64_t t = 0; t < nstep; t++) {
     if (isc[IS_STATUS] != 0) break;  // a failed env stays put (the reference panics)
     if (ctrl_seq) X::par(M.nu, [&](int k) { lds[M.L.ctrl + k] = ctrl_seq[(rel * nstep + t) * M.nu + k]; });
     if (!step_env<X>(M, S, E)) break;
+    cost += isc[IS_ITERS];
     if (qpos_out) X::par(M.nq, [&](int k) { qpos_out[(rel * nstep + t) * M.nq + k] = lds[M.L.qpos + k]; });
   }
   // poses are not refreshed here: they are derived from qpos on demand (k_kin)
@@ -1924,6 +1926,7 @@ PHD void run_env(const MT& M, const DState& S, Env& E, int64_t nstep, const Real
     S.status[e] = isc[IS_STATUS];
     S.iters[e] = isc[IS_ITERS];
     S.nrows[e] = isc[IS_NROWS];
+    S.cost[e] = cost;
     S.time[e] += (Real)nstep * M.timestep;
   });
 }

@@ -126,9 +126,10 @@ __global__ void __launch_bounds__(64) k_step_ws(const DModel* __restrict__ Mp, c
 }
 
 // Contact-count bucketing for k_step_ws, one workgroup: envs [begin, end)
-// keyed by their last step's Newton iterations and contacts, grouped by key,
-// costliest key first, into order (offsets from begin). Also resets the
-// work-stealing counter for the launch that follows on the same stream.
+// keyed by the Newton iterations of their last launch (summed over its steps)
+// and their last contact count, grouped by key, costliest key first, into
+// order (offsets from begin). Also resets the work-stealing counter for the
+// launch that follows on the same stream.
 constexpr int BUCKETS = 64;
 static __global__ void __launch_bounds__(1024) k_bucket(const DState* __restrict__ Sp, int64_t begin, int64_t end,
                                                  int* __restrict__ order, unsigned* counter) {
@@ -140,11 +141,11 @@ static __global__ void __launch_bounds__(1024) k_bucket(const DState* __restrict
   if (t < BUCKETS) hist[t] = 0;
   __syncthreads();
   auto key = [&](int64_t e) {
-    // iterations dominate a step's cost; contacts break ties within them
-    int it = S.iters[e], nc = S.ncon[e];
-    it = it < 0 ? 0 : (it > 15 ? 15 : it);
-    nc = nc < 0 ? 0 : (nc > 3 ? 3 : nc);
-    return it * 4 + nc;
+    // iterations dominate an env's cost; contacts break ties within them
+    int it = S.cost[e], nc = S.ncon[e];
+    it = it < 0 ? 0 : (it > 31 ? 31 : it);
+    nc = nc < 0 ? 0 : (nc > 1 ? 1 : nc);
+    return it * 2 + nc;
   };
   for (int64_t i = t; i < n; i += blockDim.x) atomicAdd(&hist[key(begin + i)], 1u);
   __syncthreads();

@@ -76,7 +76,7 @@ struct emu_sim {
   std::vector<double> qpos, qvel, ctrl, qacc, applied, time, gxpos, gxmat, xpos, xmat, con, pcon, plambda, stage, gjac,
       grow, islf, gmj;
   std::vector<int> isli;
-  std::vector<int> ncon, iters, nrows, status, cgeom, pcgeom, npcon;
+  std::vector<int> ncon, iters, nrows, status, cgeom, pcgeom, npcon, cost;
   std::vector<double> lds;
   // learned warm start
   std::vector<float> mlp_P, mlp_feat, mlp_pred, mlp_tgt;
@@ -233,6 +233,7 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   s->iters.assign(nn, 0);
   s->nrows.assign(nn, 0);
   s->status.assign(nn, 0);
+  s->cost.assign(nn, 0);
   s->npcon.assign(nn, 0);
   DState& S = s->S;
   S.n = n;
@@ -250,6 +251,7 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   S.iters = s->iters.data();
   S.nrows = s->nrows.data();
   S.status = s->status.data();
+  S.cost = s->cost.data();
   S.con = s->con.data();
   S.cgeom = s->cgeom.data();
   S.pcon = s->pcon.data();
