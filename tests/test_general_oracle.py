@@ -27,6 +27,11 @@ def rx(a):
     return np.array([[1, 0, 0], [0, c, -s], [0, s, c]]).ravel()
 
 
+def ry(a):
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]]).ravel()
+
+
 def rz(a):
     c, s = np.cos(a), np.sin(a)
     return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]]).ravel()
@@ -129,3 +134,31 @@ def test_general_box_contact_matches_the_plane_primitive():
     p = po.collide(0, [0, 0, 0], I3, [5, 5, .1], CYL, cpos, I3, [.03, .06, 0])
     assert len(g) == 4 and len(p) >= 1
     assert np.allclose(g["depth"], p["depth"].max(), atol=1e-12)
+
+
+def test_crossed_cylinders_side_to_side():
+    """Perpendicular axes: the sides touch at one point, depth = r1 + r2 - d."""
+    r1, r2, d = 0.04, 0.03, 0.068
+    h = po.collide(CYL, [0, 0, 0], rx(np.pi / 2), [r1, .1, 0], CYL, [0, 0, d], ry(np.pi / 2), [r2, .1, 0])
+    assert len(h) >= 1
+    assert np.allclose(h["depth"], r1 + r2 - d, atol=2e-9)
+    # both sides curved: the normal within the EPA stop's angle, as above
+    assert np.allclose(h["normal"], [[0, 0, 1]] * len(h), atol=3e-4)
+    assert np.allclose(h["pos"][:, :2], 0, atol=1e-4)
+
+
+@pytest.mark.parametrize("where", ["below_cap", "inside_side"])
+def test_sphere_centre_inside_cylinder(where):
+    """Deep penetration (the centre inside the cylinder): EPA finds the
+    nearest face; depth = distance to it + radius."""
+    r = 0.01
+    if where == "below_cap":
+        c, depth, n = [0.0, 0.0, 0.055], (0.06 - 0.055) + r, [0, 0, 1]
+    else:
+        c, depth, n = [0.035, 0.0, 0.0], (0.04 - 0.035) + r, [1, 0, 0]
+    h = po.collide(CYL, [0, 0, 0], I3, [.04, .06, 0], SPH, c, I3, [r, 0, 0])
+    assert len(h) == 1
+    assert h[0]["depth"] == pytest.approx(depth, abs=2e-9)
+    # EPA stops once the support gains <= 1e-9 m; on the curved side that
+    # leaves the normal's angle within sqrt(2 * 1e-9 / 0.04) ~ 2.2e-4 rad
+    assert np.allclose(h[0]["normal"], n, atol=3e-4)
