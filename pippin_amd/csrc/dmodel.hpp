@@ -81,6 +81,7 @@ struct Layout {
   int ndouble;  // total doubles
   // int region (offsets in ints, after the doubles)
   int i_iscal;
+  int i_isl;  // Newton island bookkeeping (row block masks, gathered indices, islands); several dof blocks only
   int nint;
   int bytes;  // per env, 16-byte aligned
 };
@@ -153,7 +154,6 @@ struct DState {
   gp<Real> grow;      // n x nrmax x 10
   gp<Real> gmj;       // n x nrmax x nv: PGS M^-1 J^T rows
   gp<Real> isl_f;     // n x isl_fstride: gathered island problems
-  gp<int> isl_i;      // n x isl_istride: row block masks, gathered indices, islands
   // learned warm start: per env, the first MLP_CL touching contacts of the last step
   gp<float> mlp_feat;  // n x MLP_CL x MLP_NF features
   gp<float> mlp_pred;  // n x MLP_CL x MLP_NO predicted log1p(lambda / D)

@@ -399,14 +399,14 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
   M.nrmax = std::max(4 * M.ncmax + 2 * hm.nlimited + d.neq, 1);
   // island scratch (newton.rs:378-473), only when several dof blocks exist
   M.isl_fstride = M.nblock > 1 ? d.nv * d.nv + M.nrmax * (d.nv + 2) + 4 * d.nv : 0;
-  M.isl_istride = M.nblock > 1 ? 2 * M.nrmax + d.nv + 2 * M.nblock + 1 : 0;
+  M.isl_istride = M.nblock > 1 ? 2 * M.nrmax + d.nv + 2 * M.nblock + 1 : 0;  // ints, in LDS (Layout::i_isl)
   // Rows beyond nrl live in HBM. Keep up to 32 in LDS, fewer if that keeps
   // the per-env working set within 10 KiB (four 4-env workgroups per CU).
   M.nrl = std::min(M.nrmax, 32);
-  M.L = make_layout(hm, M.nrl, M.npair, M.ncmax);
+  M.L = make_layout(hm, M.nrl, M.npair, M.ncmax, M.isl_istride);
   while (M.L.bytes > 10 * 1024 && M.nrl > 16) {
     M.nrl -= 4;
-    M.L = make_layout(hm, M.nrl, M.npair, M.ncmax);
+    M.L = make_layout(hm, M.nrl, M.npair, M.ncmax, M.isl_istride);
   }
 
 }
@@ -428,7 +428,7 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
 //     touching-contact list, constraint rows (row positions and diagonals share
 //     storage with J*a and J*p, which Newton fills only after the rows are built).
 // Poses for export are recomputed by the final kinematics pass of a launch.
-phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax) {
+phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax, int isl_ints) {
   phk::Layout L;
   const ph_model_desc& d = hm.d;
   int nq = d.nq, nv = d.nv, nu = d.nu, nb = d.nbody, ng = d.ngeom;
@@ -508,6 +508,8 @@ phk::Layout make_layout(const phh::HostModel& hm, int nrl, int npair, int ncmax)
   int oi = 0;
   L.i_iscal = oi;
   oi += 16;
+  L.i_isl = oi;
+  oi += isl_ints;
   L.nint = oi;
   // 16-byte aligned so that envs can be packed side by side in one workgroup
   L.bytes = (int)((L.ndouble * sizeof(double) + L.nint * sizeof(int) + 15) / 16 * 16);

@@ -55,7 +55,7 @@ bool has_implicit_terms(const HostModel& m, const double* dof_damping, const dou
 // Candidate-pair filter pieces (model.rs:430-452), masks excluded.
 bool pair_static_ok(const HostModel& m, int g1, int g2);
 // LDS layout for one env: nrl rows kept in LDS, npair candidate pairs, ncmax contacts.
-phk::Layout make_layout(const HostModel& m, int nrl, int npair, int ncmax);
+phk::Layout make_layout(const HostModel& m, int nrl, int npair, int ncmax, int isl_ints);
 void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk::DModel& M);
 
 }  // namespace phh

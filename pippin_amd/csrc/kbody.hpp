@@ -1273,9 +1273,10 @@ PHD int newton_solve(const MT& M, const DState& S, Env& E, int nr, const Real* w
     return newton_dense<X>(M, E, P);
   };
   if (M.nblock <= 1) return whole();
-  // per-env island scratch in HBM
+  // per-env island scratch: floats in HBM (gathered into LDS below when an
+  // island fits), the bookkeeping ints in the env's LDS int region
   Real* fs = (S.isl_f + E.e * (int64_t)M.isl_fstride).p;
-  int* is = (S.isl_i + E.e * (int64_t)M.isl_istride).p;
+  int* is = E.si + M.L.i_isl;
   int* rmask = is;                 // nrmax: blocks each row touches
   int* sidx = rmask + M.nrmax;     // nv: gathered dofs
   int* sridx = sidx + nv;          // nrmax: gathered rows

@@ -217,7 +217,7 @@ struct ph_sim {
   // state buffers
   DBuf<double> s_qpos, s_qvel, s_ctrl, s_qacc, s_applied, s_time, s_gxpos, s_gxmat, s_xpos, s_xmat, s_con, s_pcon,
       s_plambda, s_stage, s_gjac, s_grow, s_islf, s_gmj;
-  DBuf<int> s_ncon, s_iters, s_nrows, s_status, s_cgeom, s_pcgeom, s_npcon, s_isli, s_cost;
+  DBuf<int> s_ncon, s_iters, s_nrows, s_status, s_cgeom, s_pcgeom, s_npcon, s_cost;
   DBuf<int64_t> d_ids;
   DBuf<DModel> d_model;  // device copies of M and S, read through the scalar cache
   DBuf<DState> d_state;
@@ -502,7 +502,6 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   HIPCHK(s->s_grow.alloc(n * (int64_t)M.nrmax * 10));
   HIPCHK(s->s_islf.alloc(std::max<int64_t>(n * (int64_t)M.isl_fstride, 1)));
   HIPCHK(s->s_gmj.alloc(M.solver_kind == 1 ? n * (int64_t)M.nrmax * d.nv : 1));
-  HIPCHK(s->s_isli.alloc(std::max<int64_t>(n * (int64_t)M.isl_istride, 1)));
   for (auto* b : {&s->s_ncon, &s->s_iters, &s->s_nrows, &s->s_status, &s->s_npcon, &s->s_cost}) {
     HIPCHK(b->alloc(n));
     HIPCHK(b->zero());
@@ -533,7 +532,6 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   S.grow = s->s_grow.p;
   S.isl_f = s->s_islf.p;
   S.gmj = s->s_gmj.p;
-  S.isl_i = s->s_isli.p;
 
   // invweight0 (forward::set_const) for the base variant, then poses (PippinCpu::new runs kinematics)
   if (int rc = run_setconst(s.get(), {0})) return rc;

@@ -75,7 +75,6 @@ struct emu_sim {
   std::vector<int> v_damped;
   std::vector<double> qpos, qvel, ctrl, qacc, applied, time, gxpos, gxmat, xpos, xmat, con, pcon, plambda, stage, gjac,
       grow, islf, gmj;
-  std::vector<int> isli;
   std::vector<int> ncon, iters, nrows, status, cgeom, pcgeom, npcon, cost;
   std::vector<double> lds;
   // learned warm start
@@ -215,7 +214,6 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   s->grow.assign(nn * (int64_t)M.nrmax * 10, 0.0);
   s->islf.assign(std::max<int64_t>(nn * (int64_t)M.isl_fstride, 1), 0.0);
   s->gmj.assign(M.solver_kind == 1 ? nn * (int64_t)M.nrmax * d.nv : 1, 0.0);
-  s->isli.assign(std::max<int64_t>(nn * (int64_t)M.isl_istride, 1), 0);
   // EMU_POISON_SCRATCH=1 fills the buffers the device library leaves
   // uninitialised at create (contacts, staging, row spill, island scratch)
   // with NaN / -1, so that a read before write differs from the oracle
@@ -227,7 +225,7 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
     for (auto* v : {&s->gxpos, &s->gxmat, &s->xpos, &s->xmat, &s->con, &s->pcon, &s->plambda, &s->stage, &s->gjac,
                     &s->grow, &s->islf, &s->gmj})
       std::fill(v->begin(), v->end(), dv);
-    for (auto* v : {&s->cgeom, &s->pcgeom, &s->isli}) std::fill(v->begin(), v->end(), iv);
+    for (auto* v : {&s->cgeom, &s->pcgeom}) std::fill(v->begin(), v->end(), iv);
   }
   s->ncon.assign(nn, 0);
   s->iters.assign(nn, 0);
@@ -263,7 +261,6 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   S.grow = s->grow.data();
   S.isl_f = s->islf.data();
   S.gmj = s->gmj.data();
-  S.isl_i = s->isli.data();
   s->lds.assign(M.L.bytes / sizeof(double) + 16, 0.0);
   if (std::getenv("EMU_POISON_LDS")) std::fill(s->lds.begin(), s->lds.end(), std::numeric_limits<double>::quiet_NaN());
   // set_const for the base variant
