@@ -375,13 +375,16 @@ def test_random_piles_with_general_pairs_exact(seed, solver):
     assert o.ncon().sum() > 0
 
 
-def test_arm_gripper_exact():
+@pytest.mark.parametrize("solver", [0, 1])
+def test_arm_gripper_exact(solver):
     """The 7-DoF arm with tendon-coupled gripper (pippin_amd/workloads.py
     arm_gripper_xml): implicitfast, position actuators, fixed tendon, joint
-    equality, limits and hand/cube/table contacts, on smooth random targets."""
+    equality, limits and hand/cube/table contacts, on smooth random targets;
+    Newton and PGS (whose 80-odd rows use the global row buffers)."""
     from pippin_amd.workloads import arm_gripper_xml, arm_targets
 
     arr = mjcf.load_string(arm_gripper_xml(3))
+    arr["solver_kind"] = solver
     n = 4
     o, e = OracleSim(arr, n), EmuSim(arr, n)
     tg = arm_targets(3, n, 150, int(arr["nu"]))

@@ -392,13 +392,15 @@ def test_random_piles_with_general_pairs_match(seed, solver):
     assert o.ncon().sum() > 0
 
 
-def test_arm_gripper_with_per_env_cubes_match():
+@pytest.mark.parametrize("solver", [0, 1])
+def test_arm_gripper_with_per_env_cubes_match(solver):
     """The 7-DoF arm with tendon-coupled gripper, cube size and friction
     varied per env group (copy-on-write parameter variants), 200 steps on
-    smooth random targets."""
+    smooth random targets; Newton and PGS."""
     from pippin_amd.workloads import arm_gripper_xml, arm_targets
 
     arr = mjcf.load_string(arm_gripper_xml(5))
+    arr["solver_kind"] = solver
     n = 24
     o, g = OracleSim(arr, n), HipPhysics(arr, n)
     cube = int(arr["ngeom"]) - 1
