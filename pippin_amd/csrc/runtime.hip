@@ -610,8 +610,8 @@ static unsigned ws_grid(const ph_sim* s, int K, int spec, int64_t nslots) {
 // The step without the learned warm start in the given launch shape, with
 // optional per-step controls and qpos outputs (Batch::rollout). Only kernels
 // go on the stream, so the sequence can be captured in a graph.
-static void launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t begin, int64_t end, int64_t nstep,
-                         const Real* ctrl = nullptr, Real* qpos_out = nullptr) {
+static hipError_t launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t begin, int64_t end, int64_t nstep,
+                               const Real* ctrl = nullptr, Real* qpos_out = nullptr) {
   const int K = sh.envs_per_wave, spec = sh.specialised;
   const size_t lds = (size_t)K * s->M.L.bytes;
   const DModel* mp = s->d_model.p;
@@ -619,8 +619,7 @@ static void launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t begin, in
   if (sh.schedule == PH_SCHED_GRID) {
     const unsigned grid = (unsigned)((end - begin + K - 1) / K);
     void* args[] = {&mp, &sp, &begin, &end, &nstep, &ctrl, &qpos_out};
-    (void)hipLaunchKernel(step_kernel(K, spec), dim3(grid), dim3(64), args, lds, s->stream);
-    return;
+    return hipLaunchKernel(step_kernel(K, spec), dim3(grid), dim3(64), args, lds, s->stream);
   }
   int* order = nullptr;
   unsigned* counter = s->d_counter.p;
@@ -630,10 +629,11 @@ static void launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t begin, in
   } else {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s->stream, counter);
   }
+  if (hipError_t e = hipGetLastError()) return e;
   const int64_t nslots = (end - begin + K - 1) / K;
   const int* corder = order;
   void* args[] = {&mp, &sp, &begin, &end, &nstep, &ctrl, &qpos_out, &corder, &counter};
-  (void)hipLaunchKernel(ws_kernel(K, spec), dim3(ws_grid(s, K, spec, nslots)), dim3(64), args, lds, s->stream);
+  return hipLaunchKernel(ws_kernel(K, spec), dim3(ws_grid(s, K, spec, nslots)), dim3(64), args, lds, s->stream);
 }
 
 static ph_launch_shape current_shape(const ph_sim* s) {
@@ -644,33 +644,20 @@ static ph_launch_shape current_shape(const ph_sim* s) {
   return sh;
 }
 
-// One k_step launch over [begin, end) with the env packing chosen at create.
-static void launch_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const Real* ctrl, Real* qpos_out) {
+// One step launch over [begin, end) in the current launch shape, or the
+// learned-warm-start kernel when that is on.
+static hipError_t launch_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const Real* ctrl,
+                              Real* qpos_out) {
+  if (!s->M.mlp.mode) return launch_plain(s, current_shape(s), begin, end, nstep, ctrl, qpos_out);
   const int K = s->envs_per_wg;
   const unsigned grid = (unsigned)((end - begin + K - 1) / K);
   const size_t lds = (size_t)K * s->M.L.bytes;
-  if (!s->M.mlp.mode) {
-    launch_plain(s, current_shape(s), begin, end, nstep, ctrl, qpos_out);
-    return;
+  phm::launch_step(K, s->spec, grid, lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep, ctrl, qpos_out);
+  if ((s->M.mlp.mode & MLP_LOG) && nstep > 0) {
+    s->log_begin = begin;
+    s->log_end = end;
   }
-  if (s->M.mlp.mode) {
-    phm::launch_step(K, s->spec, grid, lds, s->stream, s->d_model.p, s->d_state.p, begin, end, nstep, ctrl, qpos_out);
-    if ((s->M.mlp.mode & MLP_LOG) && nstep > 0) {
-      s->log_begin = begin;
-      s->log_end = end;
-    }
-  } else if (s->spec == 1 && K == 4)
-    hipLaunchKernelGGL((k_step<4, SpecAnt>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
-                       end, nstep, ctrl, qpos_out);
-  else if (K == 4)
-    hipLaunchKernelGGL((k_step<4, DModel>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
-                       end, nstep, ctrl, qpos_out);
-  else if (K == 2)
-    hipLaunchKernelGGL((k_step<2, DModel>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
-                       end, nstep, ctrl, qpos_out);
-  else
-    hipLaunchKernelGGL((k_step<1, DModel>), dim3(grid), dim3(64), lds, s->stream, s->d_model.p, s->d_state.p, begin,
-                       end, nstep, ctrl, qpos_out);
+  return hipGetLastError();
 }
 
 static bool same_shape(const ph_launch_shape& a, const ph_launch_shape& b) {
@@ -805,8 +792,8 @@ static int launch_plain_graph(ph_sim* s, const ph_launch_shape& sh, int64_t begi
     }
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  launch_plain(s, sh, begin, end, nstep);
-  hipError_t launch_err = hipGetLastError();
+  hipError_t launch_err = launch_plain(s, sh, begin, end, nstep);
+  if (launch_err == hipSuccess) launch_err = hipGetLastError();
   hipError_t cap_err = hipStreamEndCapture(s->stream, &graph);
   HIPCHK(launch_err);
   HIPCHK(cap_err);
@@ -836,11 +823,11 @@ static int timed_launch(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, co
   const bool trial = plain && s->tune_rounds > 0 && nstep > 0;
   const ph_launch_shape sh = trial ? s->tune_cand[s->tune_next] : current_shape(s);
   HIPCHK(hipEventRecord(s->ev0, s->stream));
-  if (!plain) launch_step(s, begin, end, nstep, ctrl, qpos_out);
+  if (!plain) HIPCHK(launch_step(s, begin, end, nstep, ctrl, qpos_out));
   else if (s->use_graphs && !trial && !ctrl && !qpos_out) {
     if (int rc = launch_plain_graph(s, sh, begin, end, nstep)) return rc;
   } else
-    launch_plain(s, sh, begin, end, nstep, ctrl, qpos_out);
+    HIPCHK(launch_plain(s, sh, begin, end, nstep, ctrl, qpos_out));
   s->poses_stale = true;
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(s->ev1, s->stream));
