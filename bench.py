@@ -62,8 +62,10 @@ def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
 
     The same actions as the timed GPU window (after the same warmup), for the
     first `sample` envs, step-synchronous over the range with Pippin's chunked
-    thread pool. Also counts the FP64 operations of the reference algorithm on
-    that window (op-counting build, 256 envs), the roofline's numerator.
+    thread pool (`value`), and again in Batch::rollout mode (`rollout_value`,
+    which must end in the same state). Also counts the FP64 operations of the
+    reference algorithm on that window (op-counting build, 256 envs), the
+    roofline's numerator.
     """
     from oracle import pyoracle
     from oracle.pyoracle import OracleSim, ops_per_env_step
@@ -83,11 +85,23 @@ def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
         sim.step(0, sample, substeps, threads=threads)
     dt = time.perf_counter() - t0
     env_steps = sample * substeps * (nact - warmup)
+    # Batch::rollout mode (batch.rs:46-59) over the same window: each thread
+    # takes its chunk of envs through every step, controls per physics step
+    ro = OracleSim(arrays, sample)
+    ro.set("qpos", 0, sample, q0[:sample])
+    per_step = np.repeat(np.transpose(ctrl[:, :sample], (1, 0, 2)), substeps, axis=1)  # envs x steps x nu
+    ro.rollout(np.ascontiguousarray(per_step[:, :warmup * substeps]), threads=threads, want_qpos=False)
+    r0 = time.perf_counter()
+    ro.rollout(np.ascontiguousarray(per_step[:, warmup * substeps:]), threads=threads, want_qpos=False)
+    rdt = time.perf_counter() - r0
+    same = bool(np.array_equal(ro.get("qpos"), sim.get("qpos")))
     ops = ops_per_env_step(arrays, q0[:256], ctrl[:, :256], substeps, skip=warmup)
     return {
         "value": env_steps / dt,
         "unit": "env-steps/s",
         "cores": threads,
+        "rollout_value": env_steps / rdt,
+        "rollout_equals_step_loop": same,
         "kind": "port",
         "sample": f"Ant, envs 0..{sample} of {n}, the {nact - warmup} timed actions x {substeps} substeps "
                   f"after the same {warmup} warmup actions ({env_steps} env-steps in {dt:.1f} s), "
