@@ -181,3 +181,26 @@ def test_rollout_matches_under_every_shape_and_while_autotuning():
     for shape, res in zip(shapes[1:], outs[1:]):
         for a, b in zip(outs[0], res):
             assert np.array_equal(a, b), shape
+
+
+def test_autotune_cache_is_keyed_by_env_count(assets, tmp_path):
+    """A shape tuned for one env count is not taken for another: the key holds
+    the model's sizes, the env count and the device."""
+    arr = mjcf.load_file(assets / "ant.xml")
+    cache = tmp_path / "shapes.txt"
+    cache.write_text("v1 not-this-model : 1 0 0\n")
+    g = HipPhysics(arr, 32)
+    g.autotune(rounds=1, cache_path=cache)
+    assert g.autotune_pending()
+    while g.autotune_pending():
+        g.step(0, 32, 1)
+    lines = cache.read_text().strip().splitlines()
+    assert len(lines) == 2 and " n32 " in lines[1]
+    h = HipPhysics(arr, 48)
+    h.autotune(rounds=1, cache_path=cache)
+    assert h.autotune_pending()
+    k = HipPhysics(arr, 32)
+    k.autotune(rounds=1, cache_path=cache)
+    assert not k.autotune_pending() and k.launch_shape() == g.launch_shape()
+    for s in (g, h, k):
+        s.close()
