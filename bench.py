@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_step launch of the default workload (16384 envs x 4 substeps),
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a known-size copy
 # (profiles/k_step_ws_ant16k.md); other workloads report null.
-TRAFFIC_BYTES_PER_LAUNCH = 40.2e6
+TRAFFIC_BYTES_PER_LAUNCH = 44.1e6
 
 
 def log(*a):
