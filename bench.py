@@ -10,6 +10,15 @@ AsyncEnv drives a backend). `value` counts physics steps: env-steps/s =
 envs x substeps x actions / s. The controls for all actions are resident in HBM
 before the timed region; each action copies its slice device-to-device.
 
+The timed window starts at action 250 of the 1000-action run (physics step
+1000), whatever --steps and --warmup are: the first 250 actions are an untimed
+pre-roll (the launch-shape autotuner runs inside it), so the Ant has left its
+start pose and walks with steady ground contact, then come --warmup untimed and
+--steps timed actions. The line checks itself: after the timed actions the
+state of all 16384 envs is compared with the oracle stepped over the same
+actions from the same start (`max_abs_qpos_err`, `ncon_mismatches`), and the
+run exits with status 1 when the error exceeds the metric's 1e-9.
+
 Printed extras: `roofline` for the step kernel (counted FP64 operations per
 env-step of the reference algorithm, measured by the oracle's op-counting build,
 over the kernel time from hipEvents on the library's stream) and `cpu_baseline`
@@ -39,6 +48,9 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a known-size copy
 # (profiles/k_step_ws_ant16k.md); other workloads report null.
 TRAFFIC_BYTES_PER_LAUNCH = 44.1e6
+TRAFFIC_SOURCE = "profiles/k_step_ws_ant16k.md"
+PREROLL_ACTIONS = 250  # untimed: the timed window starts at action 250 of 1000 (steady contact)
+ERR_BOUND = 1e-9
 
 
 def log(*a):
@@ -57,15 +69,17 @@ def controls(n: int, nu: int, nact: int, seed: int) -> np.ndarray:
     return rng.uniform(-1.0, 1.0, size=(nact, n, nu))
 
 
-def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
-    """Time the oracle on a bounded sample of the same workload on the host cores.
+def cpu_baseline(arrays, q0, ctrl, substeps: int, t_begin: int, sample: int):
+    """Time the oracle on a bounded sample of the same workload on the host cores,
+    and return its end state for the self-check of the GPU's.
 
-    The same actions as the timed GPU window (after the same warmup), for the
-    first `sample` envs, step-synchronous over the range with Pippin's chunked
-    thread pool (`value`), and again in Batch::rollout mode (`rollout_value`,
-    which must end in the same state). Also counts the FP64 operations of the
-    reference algorithm on that window (op-counting build, 256 envs), the
-    roofline's numerator.
+    The first `sample` envs (all 16384 by default) from the same start with the
+    same controls: actions [0, t_begin) untimed, then the timed GPU window's
+    actions, step-synchronous over the range with Pippin's chunked thread pool
+    (`value`); again in Batch::rollout mode (`rollout_value`, which must end in
+    the same state). Also counts the FP64 operations of the reference
+    algorithm on the window (op-counting build, 256 envs), the roofline's
+    numerator.
     """
     from oracle import pyoracle
     from oracle.pyoracle import OracleSim, ops_per_env_step
@@ -76,26 +90,27 @@ def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
     sample = min(n, sample)
     sim = OracleSim(arrays, sample)
     sim.set("qpos", 0, sample, q0[:sample])
-    for a in range(warmup):
+    for a in range(t_begin):
         sim.set("ctrl", 0, sample, ctrl[a, :sample])
         sim.step(0, sample, substeps, threads=threads)
     t0 = time.perf_counter()
-    for a in range(warmup, nact):
+    for a in range(t_begin, nact):
         sim.set("ctrl", 0, sample, ctrl[a, :sample])
         sim.step(0, sample, substeps, threads=threads)
     dt = time.perf_counter() - t0
-    env_steps = sample * substeps * (nact - warmup)
+    end = {"qpos": sim.get("qpos"), "qvel": sim.get("qvel"), "ncon": sim.ncon()}
+    env_steps = sample * substeps * (nact - t_begin)
     # Batch::rollout mode (batch.rs:46-59) over the same window: each thread
     # takes its chunk of envs through every step, controls per physics step
     ro = OracleSim(arrays, sample)
     ro.set("qpos", 0, sample, q0[:sample])
     per_step = np.repeat(np.transpose(ctrl[:, :sample], (1, 0, 2)), substeps, axis=1)  # envs x steps x nu
-    ro.rollout(np.ascontiguousarray(per_step[:, :warmup * substeps]), threads=threads, want_qpos=False)
+    ro.rollout(np.ascontiguousarray(per_step[:, :t_begin * substeps]), threads=threads, want_qpos=False)
     r0 = time.perf_counter()
-    ro.rollout(np.ascontiguousarray(per_step[:, warmup * substeps:]), threads=threads, want_qpos=False)
+    ro.rollout(np.ascontiguousarray(per_step[:, t_begin * substeps:]), threads=threads, want_qpos=False)
     rdt = time.perf_counter() - r0
-    same = bool(np.array_equal(ro.get("qpos"), sim.get("qpos")))
-    ops = ops_per_env_step(arrays, q0[:256], ctrl[:, :256], substeps, skip=warmup)
+    same = bool(np.array_equal(ro.get("qpos"), end["qpos"]))
+    ops = ops_per_env_step(arrays, q0[:256], ctrl[:, :256], substeps, skip=t_begin)
     return {
         "value": env_steps / dt,
         "unit": "env-steps/s",
@@ -103,11 +118,11 @@ def cpu_baseline(arrays, q0, ctrl, substeps: int, warmup: int, sample: int):
         "rollout_value": env_steps / rdt,
         "rollout_equals_step_loop": same,
         "kind": "port",
-        "sample": f"Ant, envs 0..{sample} of {n}, the {nact - warmup} timed actions x {substeps} substeps "
-                  f"after the same {warmup} warmup actions ({env_steps} env-steps in {dt:.1f} s), "
+        "sample": f"Ant, envs 0..{sample} of {n}, the {nact - t_begin} timed actions x {substeps} substeps "
+                  f"after the same {t_begin} untimed actions ({env_steps} env-steps in {dt:.1f} s), "
                   f"Physics::step over the range, chunk = n/(4 x {threads} threads), "
                   f"oracle built {'-march=x86-64-v3' if tuned else 'for plain x86-64'} -O3 -ffp-contract=off",
-    }, ops
+    }, ops, end
 
 
 def copy_calibration(gib: float) -> None:
@@ -383,9 +398,8 @@ def main():
 
     arrays = load_model(ROOT / "pippin_amd" / "assets" / "ant.xml")
     n, sub = args.envs, args.substeps
-    TUNE_MAX = 96  # actions reserved for the autotuner's trial launches
-    tune = 0 if args.shape else TUNE_MAX
-    nact = tune + args.warmup + args.steps
+    pre = PREROLL_ACTIONS
+    nact = pre + args.warmup + args.steps
     sim = HipPhysics(arrays, n)
     if args.shape:
         k, sp, sc = args.shape.split(",")
@@ -398,22 +412,24 @@ def main():
     dctrl = torch.from_numpy(ctrl).to("cuda")  # resident in HBM before timing
     base = dctrl.data_ptr()
     stride = n * nu * 8
-    log(f"bench: {n} envs x {sub} substeps, {args.warmup} warmup + {args.steps} timed actions")
+    log(f"bench: {n} envs x {sub} substeps, {pre} pre-roll + {args.warmup} warmup + {args.steps} timed actions")
 
-    # the autotuner tries every launch shape on live (untimed) actions; every
-    # shape gives the same trajectories, so this is warmup like any other
+    # Untimed pre-roll to steady contact. The autotuner tries every launch shape
+    # on its first live actions; every shape gives the same trajectories, so
+    # this is pre-roll like any other.
     a0 = 0
-    if tune:
+    if not args.shape:
         sim.autotune(rounds=args.tune_rounds)
-        while sim.autotune_pending() and a0 < tune:
+        while sim.autotune_pending() and a0 < pre:
             sim.set_device("ctrl", 0, n, base + a0 * stride)
             sim.step(0, n, sub)
             a0 += 1
         log(f"autotuned after {a0} actions: launch shape {sim.launch_shape()}")
-    for a in range(a0, a0 + args.warmup):
+    tuned_actions = a0
+    for a in range(a0, pre + args.warmup):
         sim.set_device("ctrl", 0, n, base + a * stride)
         sim.step(0, n, sub)
-    t_begin = a0 + args.warmup
+    t_begin = pre + args.warmup
     torch.cuda.synchronize()
     sim.kernel_timing(reset=True)
     t0 = time.perf_counter()
@@ -426,10 +442,11 @@ def main():
     env_steps = n * sub * args.steps
     value = env_steps / dt
     log(f"timed {args.steps} actions in {dt:.3f} s; kernel {kms / launches:.3f} ms/launch")
+    t_end = t_begin + args.steps
+    gpu_end = {"qpos": sim.get("qpos"), "qvel": sim.get("qvel"), "ncon": sim.ncon()}
 
     # Newton iterations per step and contacts, over a few more (untimed) actions
     iters, ncon = [], []
-    t_end = t_begin + args.steps
     for a in range(t_end - min(args.iter_stats, t_end), t_end):
         sim.set_device("ctrl", 0, n, base + a * stride)
         for _ in range(sub):
@@ -439,8 +456,17 @@ def main():
     iters = np.concatenate(iters) if iters else np.zeros(1)
     ncon = np.concatenate(ncon) if ncon else np.zeros(1, dtype=np.int32)
 
-    cb, ops = (None, OPS_PER_ENV_STEP_DEFAULT) if args.no_cpu else cpu_baseline(
-        arrays, q0, ctrl[:t_end], sub, t_begin, args.cpu_envs)
+    check = None
+    if args.no_cpu:
+        cb, ops = None, OPS_PER_ENV_STEP_DEFAULT
+    else:
+        cb, ops, cpu_end = cpu_baseline(arrays, q0, ctrl[:t_end], sub, t_begin, args.cpu_envs)
+        m = cpu_end["qpos"].shape[0]
+        check = {"envs_compared": m, "bound": ERR_BOUND,
+                 "max_abs_qpos_err": float(np.abs(gpu_end["qpos"][:m] - cpu_end["qpos"]).max()),
+                 "max_abs_qvel_err": float(np.abs(gpu_end["qvel"][:m] - cpu_end["qvel"]).max()),
+                 "ncon_mismatches": int((gpu_end["ncon"][:m] != cpu_end["ncon"]).sum()),
+                 "against": "the oracle stepped over the same actions from the same start"}
     # roofline of the step kernel: FP64 operations of the reference algorithm per env-step
     # (op-counting oracle on the same window) over the kernel's own time
     kernel_s = kms / 1e3 / launches
@@ -458,26 +484,38 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "Ant (assets/ant.xml) 16384 envs x 4 substeps per action, Newton, FP64, "
-                               "ctrl U[-1,1] per action, qpos0 + N(0,0.01)",
-                   "envs": n, "substeps": sub, "actions": args.steps,
+        "config": {"workload": f"Ant (assets/ant.xml) 16384 envs x 4 substeps per action, Newton, FP64, "
+                               f"ctrl U[-1,1] per action, qpos0 + N(0,0.01); timed window = actions "
+                               f"{t_begin}..{t_end} of the 1000-action run, after an untimed {pre}-action "
+                               f"pre-roll to steady contact and {args.warmup} warmup actions",
+                   "envs": n, "substeps": sub, "actions": args.steps, "window_first_action": t_begin,
                    "launch_shape": dict(zip(("envs_per_wave", "specialised", "schedule"), sim.launch_shape())),
-                   "autotune_actions": a0},
+                   "autotune_actions": tuned_actions},
         "newton_iters_per_step": {"mean": float(iters.mean()), "p50": float(np.percentile(iters, 50)),
                                   "p99": float(np.percentile(iters, 99))},
         "contacts_per_env": {"mean": float(ncon.mean()), "max": int(ncon.max())},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH if (n, sub) == (16384, 4) else None,
-                     "note": f"FP64 VALU work (MI355X FP64 peak, vector = matrix); {ops:.0f} reference FP64 ops "
-                             f"per env-step; step kernel {kernel_s * 1e3:.3f} ms per launch "
-                             f"({n} envs x {sub} steps); traffic = HBM bytes per launch (rocprof FETCH_SIZE + "
-                             f"WRITE_SIZE, copy-calibrated), far below the compute time at 8 TB/s"},
+                     "traffic_source": f"{TRAFFIC_SOURCE} (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this "
+                                       f"command, copy-calibrated); not measured inside this run",
+                     "note": f"the step kernel issues no MFMA: it is bound by FP64 VALU issue and the latency "
+                             f"of dependent chains (MI355X FP64 peak, vector = matrix); {ops:.0f} reference "
+                             f"FP64 ops per env-step; step kernel {kernel_s * 1e3:.3f} ms per launch "
+                             f"({n} envs x {sub} steps)"},
     }
+    if check is not None:
+        line["max_abs_qpos_err"] = check["max_abs_qpos_err"]
+        line["ncon_mismatches"] = check["ncon_mismatches"]
+        line["self_check"] = check
     if cb is not None:
         line["cpu_baseline"] = cb
         line["gpu_over_cpu"] = value / cb["value"]
     print(json.dumps(line), flush=True)
+    if check is not None and (check["max_abs_qpos_err"] > ERR_BOUND or check["ncon_mismatches"] > 0):
+        log(f"self-check failed: qpos error {check['max_abs_qpos_err']:.3e} (bound {ERR_BOUND}), "
+            f"{check['ncon_mismatches']} envs with other contact counts")
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -45,8 +45,11 @@ enum { PH_GEOM_PLANE = 0, PH_GEOM_SPHERE = 1, PH_GEOM_CAPSULE = 2, PH_GEOM_BOX =
        PH_GEOM_CYLINDER = 4, PH_GEOM_MESH = 5 };
 enum { PH_INTEGRATOR_EULER = 0, PH_INTEGRATOR_IMPLICITFAST = 1 };
 enum { PH_SOLVER_NEWTON = 0, PH_SOLVER_PGS = 1 };
-/* `Field` of crates/pippin-env/src/lib.rs:22-27. */
-enum { PH_FIELD_QPOS = 0, PH_FIELD_QVEL = 1, PH_FIELD_CTRL = 2 };
+/* `Field` of crates/pippin-env/src/lib.rs:22-27, then the rest of `Data`'s
+ * carried state (crates/pippin/src/data.rs:8-80) for checkpoint/resume:
+ * qacc (Newton's warm start, nv wide), qfrc_applied (nv) and time (1). */
+enum { PH_FIELD_QPOS = 0, PH_FIELD_QVEL = 1, PH_FIELD_CTRL = 2,
+       PH_FIELD_QACC = 3, PH_FIELD_QFRC_APPLIED = 4, PH_FIELD_TIME = 5 };
 /* `Param` of crates/pippin/src/params.rs:10-39, same order. */
 enum {
   PH_PARAM_GEOM_FRICTION = 0, PH_PARAM_GEOM_SIZE, PH_PARAM_GEOM_RGBA, PH_PARAM_GEOM_CONTYPE,
@@ -184,10 +187,13 @@ int pippin_hip_dims(const ph_sim* s, ph_dims* out);
 int64_t pippin_hip_num_envs(const ph_sim* s);
 
 /* ---- Physics::step / reset / get / set / poses (lib.rs:116-128) ---- */
-/* Advance envs [begin, end) by nstep steps, ctrl held; then refresh poses. */
+/* Advance envs [begin, end) by nstep steps, ctrl held. Poses are a function of
+ * qpos and are recomputed by the next pippin_hip_poses call (what PippinCpu's
+ * kinematics rerun after step amounts to, pippin_cpu.rs:107-110). */
 int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep);
 int pippin_hip_reset(ph_sim* s, const int64_t* envs, int64_t n);
-/* `out` / `src` hold (end - begin) x width doubles, width = dims.{nq,nv,nu}. */
+/* `out` / `src` hold (end - begin) x width doubles, width = dims.{nq,nv,nu} for
+ * qpos/qvel/ctrl, nv for qacc and qfrc_applied, 1 for time. */
 int pippin_hip_get(const ph_sim* s, int32_t field, int64_t begin, int64_t end, double* out, int64_t len);
 int pippin_hip_set(ph_sim* s, int32_t field, int64_t begin, int64_t end, const double* src, int64_t len);
 /* Geom poses (envs x ngeom) and camera poses (envs x ncam). */
@@ -219,6 +225,23 @@ int pippin_hip_invweight(const ph_sim* s, int64_t begin, int64_t end, double* do
 /* Number of candidate collision pairs and their (g1, g2) list (compile-time filtering). */
 int pippin_hip_collision_pairs(const ph_sim* s, int64_t env, int32_t* pairs, int32_t cap, int32_t* n);
 
+/* ---- checkpoint/resume of the warm-start state ----
+ * The reference keeps no such entry point; these expose what a resumed run
+ * needs beyond the PH_FIELD_* state to continue bit for bit: PGS's contact
+ * history, Scratch::prev_contacts and prev_lambda (crates/pippin/src/data.rs:60-80,
+ * matched by solver.rs:253-289, cleared by Data::reset, data.rs:122-131).
+ * Newton's warm start is PH_FIELD_QACC. */
+/* Contacts of history one env can hold (the per-env contact capacity). */
+int pippin_hip_history_capacity(const ph_sim* s, int32_t* cap);
+/* counts: (end - begin); contacts: (end - begin) x cap; lambda: (end - begin) x cap x 3
+ * (normal, tangent 1, tangent 2 impulse of each contact). get: PH_ERR_SIZE if an env
+ * holds more than cap contacts. set: validates every env first (count within the
+ * capacity, geom ids in range, finite values) and changes nothing on an error. */
+int pippin_hip_get_history(const ph_sim* s, int64_t begin, int64_t end, int32_t cap, int32_t* counts,
+                           ph_contact* contacts, double* lambda);
+int pippin_hip_set_history(ph_sim* s, int64_t begin, int64_t end, int32_t cap, const int32_t* counts,
+                           const ph_contact* contacts, const double* lambda);
+
 /* Device arithmetic check: for each x[i], out[4i..4i+3] = (sin x, cos x, sqrt|x|, 1/x)
  * as the kernels compute them (sin/cos correctly rounded). No counterpart in the
  * reference; it lets callers confirm the device math matches the host bit for bit. */
@@ -229,10 +252,13 @@ int pippin_hip_selftest_math(const double* x, int64_t n, double* out, int32_t de
  * for envs [begin, end). Runs on the library's stream and synchronises. */
 int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, int64_t nstep, const double* ctrl,
                               double* qpos_out);
-/* Device pointers of the resident state (envs x width), for zero-copy callers. */
+/* Device pointers of the resident state (envs x width), for zero-copy callers.
+ * Once a qpos pointer has been handed out, pippin_hip_poses recomputes poses on
+ * every call, since writes through the pointer are not seen by the library. */
 int pippin_hip_state_device(ph_sim* s, int32_t field, double** ptr);
 /* Copy `field` (PH_FIELD_QVEL or PH_FIELD_CTRL) for envs [begin, end) from a DEVICE
- * buffer, ordered on the library stream before the next step (no host sync). */
+ * buffer, ordered on the library stream before the next step (no host sync);
+ * pippin_hip_get and pippin_hip_set wait for it. */
 int pippin_hip_set_device(ph_sim* s, int32_t field, int64_t begin, int64_t end, const double* src);
 /* Accumulated device time of the step kernel (hipEvents around each launch on the
  * library stream) and the number of launches since the last reset. */

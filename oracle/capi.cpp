@@ -118,14 +118,30 @@ int orc_reset(orc_sim* s, const int64_t* ids, int64_t n) {
   return PH_OK;
 }
 
-static std::vector<Real>& field_ref(Data& d, int f) {
-  return f == PH_FIELD_QPOS ? d.qpos : (f == PH_FIELD_QVEL ? d.qvel : d.ctrl);
+// Data fields by PH_FIELD_* (data.rs:8-80); time is a one-element view.
+// Returns false for an unknown field.
+static bool field_ptr(Data& d, int f, Real** p, size_t* width) {
+  std::vector<Real>* v = nullptr;
+  switch (f) {
+    case PH_FIELD_QPOS: v = &d.qpos; break;
+    case PH_FIELD_QVEL: v = &d.qvel; break;
+    case PH_FIELD_CTRL: v = &d.ctrl; break;
+    case PH_FIELD_QACC: v = &d.qacc; break;
+    case PH_FIELD_QFRC_APPLIED: v = &d.qfrc_applied; break;
+    case PH_FIELD_TIME: *width = 1; *p = &d.time; return true;
+    default: return false;
+  }
+  *width = v->size();
+  *p = v->data();
+  return true;
 }
 
 int orc_get(orc_sim* s, int field, int64_t begin, int64_t end, double* out) {
   for (int64_t i = begin; i < end; i++) {
-    std::vector<Real>& v = field_ref(s->envs[(size_t)i], field);
-    std::memcpy(out + (i - begin) * v.size(), v.data(), v.size() * sizeof(double));
+    size_t w = 0;
+    Real* v = nullptr;
+    if (!field_ptr(s->envs[(size_t)i], field, &v, &w)) return PH_ERR_ARG;
+    if (w) std::memcpy(out + (i - begin) * w, v, w * sizeof(double));
   }
   return PH_OK;
 }
@@ -134,9 +150,70 @@ int orc_get(orc_sim* s, int field, int64_t begin, int64_t end, double* out) {
 int orc_set(orc_sim* s, int field, int64_t begin, int64_t end, const double* src) {
   for (int64_t i = begin; i < end; i++) {
     Data& d = s->envs[(size_t)i];
-    std::vector<Real>& v = field_ref(d, field);
-    std::memcpy(v.data(), src + (i - begin) * v.size(), v.size() * sizeof(double));
+    size_t w = 0;
+    Real* v = nullptr;
+    if (!field_ptr(d, field, &v, &w)) return PH_ERR_ARG;
+    if (w) std::memcpy(v, src + (i - begin) * w, w * sizeof(double));
     if (field == PH_FIELD_QPOS) kinematics(*s->models[(size_t)i], d);
+  }
+  return PH_OK;
+}
+
+// Scratch::prev_contacts / prev_lambda (data.rs:60-80) of envs [begin, end):
+// counts (n), contacts (n x cap), lambda (n x cap x 3). Returns PH_ERR_SIZE
+// when an env holds more than cap.
+int orc_get_history(orc_sim* s, int64_t begin, int64_t end, int32_t cap, int32_t* counts, ph_contact* out,
+                    double* lambda) {
+  for (int64_t i = begin; i < end; i++) {
+    const Data& d = s->envs[(size_t)i];
+    if ((int64_t)d.prev_contacts.size() > cap) return PH_ERR_SIZE;
+    counts[i - begin] = (int32_t)d.prev_contacts.size();
+    for (int32_t k = 0; k < cap; k++) {
+      ph_contact& o = out[(i - begin) * cap + k];
+      double* lo = lambda + ((i - begin) * cap + k) * 3;
+      o = ph_contact{};
+      lo[0] = lo[1] = lo[2] = 0.0;
+      if (k >= counts[i - begin]) continue;
+      const Contact& c = d.prev_contacts[(size_t)k];
+      for (int t = 0; t < 3; t++) {
+        o.pos[t] = c.pos[t];
+        o.normal[t] = c.normal[t];
+        o.tangent[t] = c.tangent[t];
+        lo[t] = d.prev_lambda[(size_t)k][(size_t)t];
+      }
+      o.depth = c.depth;
+      o.geom1 = (int32_t)c.geom[0];
+      o.geom2 = (int32_t)c.geom[1];
+      o.feature = (int32_t)c.feature;
+    }
+  }
+  return PH_OK;
+}
+
+int orc_set_history(orc_sim* s, int64_t begin, int64_t end, int32_t cap, const int32_t* counts, const ph_contact* in,
+                    const double* lambda) {
+  for (int64_t i = begin; i < end; i++)
+    if (counts[i - begin] < 0 || counts[i - begin] > cap) return PH_ERR_ARG;
+  for (int64_t i = begin; i < end; i++) {
+    Data& d = s->envs[(size_t)i];
+    d.prev_contacts.clear();
+    d.prev_lambda.clear();
+    for (int32_t k = 0; k < counts[i - begin]; k++) {
+      const ph_contact& c = in[(i - begin) * cap + k];
+      Contact o;
+      for (int t = 0; t < 3; t++) {
+        o.pos[t] = c.pos[t];
+        o.normal[t] = c.normal[t];
+        o.tangent[t] = c.tangent[t];
+      }
+      o.depth = c.depth;
+      o.geom[0] = (size_t)c.geom1;
+      o.geom[1] = (size_t)c.geom2;
+      o.feature = (uint32_t)c.feature;
+      d.prev_contacts.push_back(o);
+      const double* lk = lambda + ((i - begin) * cap + k) * 3;
+      d.prev_lambda.push_back({lk[0], lk[1], lk[2]});
+    }
   }
   return PH_OK;
 }

@@ -21,7 +21,8 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "liborc.so"
 _lib = None
 
-FIELD = {"qpos": 0, "qvel": 1, "ctrl": 2}
+FIELD = {"qpos": 0, "qvel": 1, "ctrl": 2, "qacc": 3, "qfrc_applied": 4, "time": 5}
+STATE_FIELDS = ["qpos", "qvel", "ctrl", "qacc", "qfrc_applied", "time"]
 PARAM = {n: i for i, n in enumerate([
     "geom_friction", "geom_size", "geom_rgba", "geom_contype", "geom_conaffinity", "body_mass",
     "dof_damping", "actuator_gain", "actuator_bias", "body_inertia", "body_ipos", "qpos0"])}
@@ -84,6 +85,8 @@ def lib():
             "orc_quat_from_mat": [vp, vp],
             "orc_quat_integrate": [vp, vp, C.c_double, vp],
             "orc_collide": [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, C.c_double, vp, i32],
+            "orc_get_history": [vp, i64, i64, i32, vp, vp, vp],
+            "orc_set_history": [vp, i64, i64, i32, vp, vp, vp],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args
@@ -145,7 +148,36 @@ class OracleSim:
         return dict(nq=self.nq, nv=self.nv, nu=self.nu, ngeom=self.ngeom, ncam=self.ncam)
 
     def width(self, field):
-        return {"qpos": self.nq, "qvel": self.nv, "ctrl": self.nu}[field]
+        return {"qpos": self.nq, "qvel": self.nv, "ctrl": self.nu, "qacc": self.nv, "qfrc_applied": self.nv,
+                "time": 1}[field]
+
+    def get_history(self, begin=0, end=None, cap=256):
+        """Scratch::prev_contacts / prev_lambda per env: counts, contacts (n, cap), impulses (n, cap, 3)."""
+        end = self.n if end is None else end
+        counts = np.zeros(end - begin, dtype=np.int32)
+        con = np.zeros((end - begin, cap), dtype=CONTACT_DTYPE)
+        lam = np.zeros((end - begin, cap, 3))
+        self._check(self.L.orc_get_history(self.h, begin, end, cap, _p(counts), _p(con), _p(lam)))
+        return counts, con, lam
+
+    def set_history(self, begin, end, counts, contacts, lam):
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        contacts = np.ascontiguousarray(contacts, dtype=CONTACT_DTYPE)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self._check(self.L.orc_set_history(self.h, begin, end, contacts.shape[1], _p(counts), _p(contacts),
+                                           _p(lam)))
+
+    def save_state(self, begin=0, end=None):
+        end = self.n if end is None else end
+        st = {f: self.get(f, begin, end) for f in STATE_FIELDS}
+        st["history"] = self.get_history(begin, end)
+        return st
+
+    def load_state(self, st, begin=0, end=None):
+        end = self.n if end is None else end
+        for f in STATE_FIELDS:
+            self.set(f, begin, end, st[f])
+        self.set_history(begin, end, *st["history"])
 
     def step(self, begin, end, nstep=1, threads=1):
         self._check(self.L.orc_step(self.h, begin, end, nstep, threads))

@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(64) k_reset(const DModel* __restrict__ Mp, con
     S.ncon[e] = 0;
     S.npcon[e] = 0;
     S.time[e] = 0.0;
+    S.status[e] = 0;  // a frozen env (not positive definite, capacity) steps again from the reset state
   }
 }
 
@@ -197,6 +198,7 @@ struct ph_sim {
   // set(Qpos), pippin_cpu.rs:107-110, 183-185); they are recomputed for all
   // envs on the first poses() call after qpos changed.
   mutable bool poses_stale = true;
+  bool qpos_shared = false;  // a device qpos pointer was handed out (pippin_hip_state_device)
   hipStream_t stream = nullptr;
   DModel M;
   DState S;
@@ -595,6 +597,9 @@ static int field_width(const ph_sim* s, int32_t f, double** dptr) {
     case PH_FIELD_QPOS: *dptr = s->s_qpos.p; return s->hm.d.nq;
     case PH_FIELD_QVEL: *dptr = s->s_qvel.p; return s->hm.d.nv;
     case PH_FIELD_CTRL: *dptr = s->s_ctrl.p; return s->hm.d.nu;
+    case PH_FIELD_QACC: *dptr = s->s_qacc.p; return s->hm.d.nv;
+    case PH_FIELD_QFRC_APPLIED: *dptr = s->s_applied.p; return s->hm.d.nv;
+    case PH_FIELD_TIME: *dptr = s->s_time.p; return 1;
     default: return -1;
   }
 }
@@ -868,7 +873,7 @@ extern "C" int pippin_hip_set_device(ph_sim* s, int32_t field, int64_t begin, in
   if (int rc = check_range(s, begin, end)) return rc;
   double* d;
   int w = field_width(s, field, &d);
-  if (w < 0 || field == PH_FIELD_QPOS) return fail(PH_ERR_ARG, "set_device takes qvel or ctrl");
+  if (field != PH_FIELD_QVEL && field != PH_FIELD_CTRL) return fail(PH_ERR_ARG, "set_device takes qvel or ctrl");
   if (w == 0 || end == begin) return PH_OK;
   HIPCHK(hipMemcpyAsync(d + begin * w, src, (end - begin) * w * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   return PH_OK;
@@ -885,6 +890,9 @@ extern "C" int pippin_hip_rollout_device(ph_sim* s, int64_t begin, int64_t end, 
 extern "C" int pippin_hip_state_device(ph_sim* s, int32_t field, double** ptr) {
   if (!s || !ptr) return fail(PH_ERR_ARG, "null argument");
   *ptr = field == PH_FIELD_QPOS ? s->s_qpos.p : field == PH_FIELD_QVEL ? s->s_qvel.p : field == PH_FIELD_CTRL ? s->s_ctrl.p : nullptr;
+  // the caller may write qpos through the pointer at any time from now on,
+  // so poses are recomputed on every poses() call
+  if (field == PH_FIELD_QPOS) s->qpos_shared = true;
   return *ptr ? PH_OK : fail(PH_ERR_ARG, "unknown field");
 }
 
@@ -898,6 +906,9 @@ extern "C" int pippin_hip_get(const ph_sim* s, int32_t field, int64_t begin, int
   if (w < 0) return fail(PH_ERR_ARG, "unknown field");
   if (int rc = check_len("get", len, end - begin, w)) return rc;
   if (w == 0 || end == begin) return PH_OK;
+  // copies queued by set_device run on the library stream, which the null
+  // stream does not wait for
+  HIPCHK(hipStreamSynchronize(s->stream));
   HIPCHK(hipMemcpy(out, d + begin * w, (end - begin) * w * sizeof(double), hipMemcpyDeviceToHost));
   return PH_OK;
 }
@@ -909,6 +920,7 @@ extern "C" int pippin_hip_set(ph_sim* s, int32_t field, int64_t begin, int64_t e
   if (w < 0) return fail(PH_ERR_ARG, "unknown field");
   if (int rc = check_len("set", len, end - begin, w)) return rc;
   if (w == 0 || end == begin) return PH_OK;
+  HIPCHK(hipStreamSynchronize(s->stream));  // a pending set_device copy must not overwrite this one
   HIPCHK(hipMemcpy(d + begin * w, src, (end - begin) * w * sizeof(double), hipMemcpyHostToDevice));
   if (field == PH_FIELD_QPOS) s->poses_stale = true;
   return PH_OK;
@@ -936,7 +948,7 @@ extern "C" int pippin_hip_poses(const ph_sim* s, int64_t begin, int64_t end, ph_
   if (int rc = check_len("poses(cams)", ncl, end - begin, d.ncam)) return rc;
   int64_t n = end - begin;
   if (n == 0) return PH_OK;
-  if (s->poses_stale) {
+  if (s->poses_stale || s->qpos_shared) {
     HIPCHK(hipSetDevice(s->device));
     hipLaunchKernelGGL(k_kin, dim3((unsigned)s->n), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p,
                        (int64_t)0, (const int64_t*)nullptr);
@@ -1247,6 +1259,108 @@ extern "C" int pippin_hip_contacts(const ph_sim* s, int64_t env, ph_contact* out
     out[i].feature = g[3 * i + 2];
     out[i].pad = 0;
   }
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_history_capacity(const ph_sim* s, int32_t* cap) {
+  if (!s || !cap) return fail(PH_ERR_ARG, "null argument");
+  *cap = s->M.ncmax;
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_get_history(const ph_sim* s, int64_t begin, int64_t end, int32_t cap, int32_t* counts,
+                                      ph_contact* out, double* lambda) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  if (cap < 0 || !counts || (cap > 0 && (!out || !lambda))) return fail(PH_ERR_ARG, "null argument or negative capacity");
+  const int64_t n = end - begin;
+  if (n == 0) return PH_OK;
+  const int64_t nc = s->M.ncmax;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  std::vector<int> cnt(n);
+  HIPCHK(hipMemcpy(cnt.data(), s->s_npcon.p + begin, n * sizeof(int), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; i++)
+    if (cnt[i] > cap)
+      return fail(PH_ERR_SIZE, "get_history: env " + std::to_string(begin + i) + " holds " + std::to_string(cnt[i]) +
+                                   " contacts, capacity " + std::to_string(cap));
+  std::vector<double> c(n * nc * CW), l(n * nc * 3);
+  std::vector<int> g(n * nc * 3);
+  HIPCHK(hipMemcpy(c.data(), s->s_pcon.p + begin * nc * CW, c.size() * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(g.data(), s->s_pcgeom.p + begin * nc * 3, g.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(l.data(), s->s_plambda.p + begin * nc * 3, l.size() * 8, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; i++) {
+    counts[i] = cnt[i];
+    for (int64_t k = 0; k < cap; k++) {
+      ph_contact& o = out[i * cap + k];
+      double* lo = lambda + (i * cap + k) * 3;
+      if (k >= cnt[i]) {
+        o = ph_contact{};
+        lo[0] = lo[1] = lo[2] = 0.0;
+        continue;
+      }
+      const double* ci = &c[(i * nc + k) * CW];
+      for (int t = 0; t < 3; t++) {
+        o.pos[t] = ci[t];
+        o.normal[t] = ci[3 + t];
+        o.tangent[t] = ci[6 + t];
+        lo[t] = l[(i * nc + k) * 3 + t];
+      }
+      o.depth = ci[9];
+      o.geom1 = g[(i * nc + k) * 3];
+      o.geom2 = g[(i * nc + k) * 3 + 1];
+      o.feature = g[(i * nc + k) * 3 + 2];
+      o.pad = 0;
+    }
+  }
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_set_history(ph_sim* s, int64_t begin, int64_t end, int32_t cap, const int32_t* counts,
+                                      const ph_contact* in, const double* lambda) {
+  if (int rc = check_range(s, begin, end)) return rc;
+  if (cap < 0 || !counts || (cap > 0 && (!in || !lambda))) return fail(PH_ERR_ARG, "null argument or negative capacity");
+  const int64_t n = end - begin;
+  if (n == 0) return PH_OK;
+  const int64_t nc = s->M.ncmax;
+  const int ngeom = s->hm.d.ngeom;
+  // validate everything before touching any env
+  for (int64_t i = 0; i < n; i++) {
+    const std::string env = "set_history: env " + std::to_string(begin + i);
+    if (counts[i] < 0 || counts[i] > cap || counts[i] > nc)
+      return fail(PH_ERR_ARG, env + ": count " + std::to_string(counts[i]) + " outside [0, min(cap, " +
+                                  std::to_string(nc) + ")]");
+    for (int k = 0; k < counts[i]; k++) {
+      const ph_contact& c = in[i * cap + k];
+      const double* lk = lambda + (i * (int64_t)cap + k) * 3;
+      if (c.geom1 < 0 || c.geom1 >= ngeom || c.geom2 < 0 || c.geom2 >= ngeom)
+        return fail(PH_ERR_ARG, env + ": geom id out of range");
+      bool finite = std::isfinite(c.depth) && std::isfinite(lk[0]) && std::isfinite(lk[1]) && std::isfinite(lk[2]);
+      for (int t = 0; t < 3; t++)
+        finite = finite && std::isfinite(c.pos[t]) && std::isfinite(c.normal[t]) && std::isfinite(c.tangent[t]);
+      if (!finite) return fail(PH_ERR_ARG, env + ": non-finite contact or impulse");
+    }
+  }
+  std::vector<double> c(n * nc * CW, 0.0), l(n * nc * 3, 0.0);
+  std::vector<int> g(n * nc * 3, 0), cnt(counts, counts + n);
+  for (int64_t i = 0; i < n; i++)
+    for (int k = 0; k < counts[i]; k++) {
+      const ph_contact& ci = in[i * cap + k];
+      double* co = &c[(i * nc + k) * CW];
+      for (int t = 0; t < 3; t++) {
+        co[t] = ci.pos[t];
+        co[3 + t] = ci.normal[t];
+        co[6 + t] = ci.tangent[t];
+        l[(i * nc + k) * 3 + t] = lambda[(i * (int64_t)cap + k) * 3 + t];
+      }
+      co[9] = ci.depth;
+      g[(i * nc + k) * 3] = ci.geom1;
+      g[(i * nc + k) * 3 + 1] = ci.geom2;
+      g[(i * nc + k) * 3 + 2] = ci.feature;
+    }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(s->s_pcon.p + begin * nc * CW, c.data(), c.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->s_pcgeom.p + begin * nc * 3, g.data(), g.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->s_plambda.p + begin * nc * 3, l.data(), l.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->s_npcon.p + begin, cnt.data(), n * sizeof(int), hipMemcpyHostToDevice));
   return PH_OK;
 }
 

@@ -18,7 +18,9 @@ from .desc import (APPEARANCE_DTYPE, CONTACT_DTYPE, POSE_DTYPE, Desc, arrays_fro
                    ph_launch_shape, ph_model_desc, ph_options)
 
 LIB_PATH = Path(__file__).resolve().parent / "libpippin_hip.so"
-FIELD = {"qpos": 0, "qvel": 1, "ctrl": 2}
+FIELD = {"qpos": 0, "qvel": 1, "ctrl": 2, "qacc": 3, "qfrc_applied": 4, "time": 5}
+# the state a resumed run needs, in the order save_state stores it
+STATE_FIELDS = ["qpos", "qvel", "ctrl", "qacc", "qfrc_applied", "time"]
 PARAMS = ["geom_friction", "geom_size", "geom_rgba", "geom_contype", "geom_conaffinity", "body_mass",
           "dof_damping", "actuator_gain", "actuator_bias", "body_inertia", "body_ipos", "qpos0"]
 PARAM = {n: i for i, n in enumerate(PARAMS)}
@@ -97,6 +99,9 @@ def lib():
             "pippin_hip_mlp_forward": ([vp, vp, i64, vp], C.c_int),
             "pippin_hip_mlp_loss_grad": ([vp, vp, vp, i64, vp, vp], C.c_int),
             "pippin_hip_mlp_records": ([vp, i64, vp, vp, vp, vp], C.c_int),
+            "pippin_hip_history_capacity": ([vp, C.POINTER(i32)], C.c_int),
+            "pippin_hip_get_history": ([vp, i64, i64, i32, vp, vp, vp], C.c_int),
+            "pippin_hip_set_history": ([vp, i64, i64, i32, vp, vp, vp], C.c_int),
         }
         global BOUND
         BOUND = sorted(sigs)
@@ -197,10 +202,49 @@ class HipPhysics:
         return dict(nq=self.nq, nv=self.nv, nu=self.nu, ngeom=self.ngeom, ncam=self.ncam)
 
     def width(self, field):
-        return {"qpos": self.nq, "qvel": self.nv, "ctrl": self.nu}[field]
+        return {"qpos": self.nq, "qvel": self.nv, "ctrl": self.nu, "qacc": self.nv, "qfrc_applied": self.nv,
+                "time": 1}[field]
 
     def step(self, begin, end, nstep=1):
         self._check(self.L.pippin_hip_step(self.h, begin, end, nstep))
+
+    # ---- checkpoint/resume (no trait counterpart) ----
+    def history_capacity(self):
+        c = C.c_int32(0)
+        self._check(self.L.pippin_hip_history_capacity(self.h, C.byref(c)))
+        return c.value
+
+    def get_history(self, begin=None, end=None, cap=None):
+        """PGS contact history per env: counts (n,), contacts (n, cap), impulses (n, cap, 3)."""
+        begin, end = self._range(begin, end)
+        cap = self.history_capacity() if cap is None else cap
+        counts = np.zeros(end - begin, dtype=np.int32)
+        con = np.zeros((end - begin, cap), dtype=CONTACT_DTYPE)
+        lam = np.zeros((end - begin, cap, 3))
+        self._check(self.L.pippin_hip_get_history(self.h, begin, end, cap, _p(counts), _p(con), _p(lam)))
+        return counts, con, lam
+
+    def set_history(self, begin, end, counts, contacts, lam):
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        contacts = np.ascontiguousarray(contacts, dtype=CONTACT_DTYPE)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        cap = contacts.shape[1]
+        if counts.shape != (end - begin,) or contacts.shape[0] != end - begin or lam.shape != (end - begin, cap, 3):
+            raise ValueError("set_history: arrays do not match the env range and capacity")
+        self._check(self.L.pippin_hip_set_history(self.h, begin, end, cap, _p(counts), _p(contacts), _p(lam)))
+
+    def save_state(self, begin=None, end=None):
+        """Everything a resumed run needs to continue bit for bit (Data's carried state)."""
+        begin, end = self._range(begin, end)
+        st = {f: self.get(f, begin, end) for f in STATE_FIELDS}
+        st["history"] = self.get_history(begin, end)
+        return st
+
+    def load_state(self, st, begin=None, end=None):
+        begin, end = self._range(begin, end)
+        for f in STATE_FIELDS:
+            self.set(f, begin, end, st[f])
+        self.set_history(begin, end, *st["history"])
 
     def reset(self, ids):
         ids = np.ascontiguousarray(ids, dtype=np.int64)

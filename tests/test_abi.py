@@ -51,6 +51,35 @@ def test_library_loader_matches_oracle_loader(assets, name):
         assert np.array_equal(va.reshape(-1), vb.reshape(-1).astype(va.dtype)), k
 
 
+def _generated_models():
+    from pippin_amd.workloads import arm_gripper_xml, random_pile_xml
+    from tests.scenes import GENERAL_SCENES, MULTIBODY_SCENES, PRIMITIVE_SCENES, general_xml, primitive_xml
+
+    out = [(f"primitive {k}", primitive_xml(k)) for k in sorted(PRIMITIVE_SCENES)]
+    out += [(f"multibody {k}", x) for k, x in sorted(MULTIBODY_SCENES.items()) if x is not None]
+    out += [(f"general {k}", general_xml(k)) for k in sorted(GENERAL_SCENES)]
+    out += [(f"pile {s}", random_pile_xml(s)) for s in (11, 29)]
+    out += [(f"general pile {s}", random_pile_xml(s, general=True)) for s in (13, 18)]
+    out += [("arm gripper 5", arm_gripper_xml(5)), ("arm gripper 0", arm_gripper_xml(0))]
+    from tests.scenes import GRIPPER
+
+    out.append(("gripper", GRIPPER))
+    return out
+
+
+@pytest.mark.parametrize("name,xml", _generated_models(), ids=[n for n, _ in _generated_models()])
+def test_library_loader_matches_oracle_loader_on_generated_models(name, xml):
+    """Every generated model the GPU tests step (primitive pairs, multi-body
+    scenes, cylinder scenes, random piles, the arm with its gripper) compiles
+    to the same arrays through the library's loader as through the oracle's."""
+    a = physics.load_model(xml, is_xml=True)
+    b = mjcf.load_string(xml)
+    assert set(a) == set(b)
+    for k in b:
+        va, vb = np.asarray(a[k]), np.asarray(b[k])
+        assert np.array_equal(va.reshape(-1), vb.reshape(-1).astype(va.dtype)), (name, k)
+
+
 def test_loader_reports_errors():
     with pytest.raises(physics.HipError, match="."):
         physics.load_model("<mujoco><worldbody><body><joint type='nope'/></body></worldbody></mujoco>",

@@ -53,6 +53,22 @@ def _run(arr, q, n, shape, calls, seed, graphs=True, ranges=None):
     return out
 
 
+def _valid_shapes(arr, n):
+    """The shapes the library accepts for this model (LDS, specialised kernel).
+    Only set_launch_shape may refuse: a failure while stepping a valid shape
+    fails the test."""
+    g = HipPhysics(arr, n)
+    ok = []
+    for shape in SHAPES:
+        try:
+            g.set_launch_shape(*shape)
+        except HipError:
+            continue
+        ok.append(shape)
+    g.close()
+    return ok
+
+
 def _same(a, b):
     return all(np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]) for x, y in zip(a, b))
 
@@ -63,16 +79,13 @@ def test_every_shape_matches_on_ant(assets):
     ranges = [(0, n), (0, n), (3, n), (0, 9)]
     ref = _run(arr, q, n, (1, 0, "grid"), 24, 5, graphs=False, ranges=ranges)
     assert sum(int(c.sum()) for _, c in ref) > 0
-    tried = 0
-    for shape in SHAPES:
-        try:
-            got = _run(arr, q, n, shape, 24, 5, ranges=ranges)
-        except HipError:
-            continue  # not available for this model (e.g. no specialised kernel for K < 4)
+    shapes = _valid_shapes(arr, n)
+    # every generic shape, and the Ant-specialised kernel at 4 envs per wavefront
+    assert len(shapes) == 12, shapes
+    for shape in shapes:
         say("ant shape", shape)
+        got = _run(arr, q, n, shape, 24, 5, ranges=ranges)
         assert _same(ref, got), shape
-        tried += 1
-    assert tried >= 10
 
 
 @pytest.mark.parametrize("solver", [0, 1])
@@ -87,16 +100,12 @@ def test_every_shape_matches_on_a_pile(solver):
     for b in range(int(arr["nq"]) // 7):
         q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
     ref = _run(arr, q, n, None, 30, 7, graphs=False)
-    tried = 0
-    for shape in SHAPES:
-        try:
-            got = _run(arr, q, n, shape, 30, 7)
-        except HipError:
-            continue
+    shapes = _valid_shapes(arr, n)
+    assert len(shapes) >= 3, shapes  # at least one K with every schedule
+    for shape in shapes:
         say("pile shape", solver, shape)
+        got = _run(arr, q, n, shape, 30, 7)
         assert _same(ref, got), shape
-        tried += 1
-    assert tried >= 3
 
 
 def test_autotune_keeps_results_and_caches_its_choice(assets, tmp_path):

@@ -14,7 +14,7 @@ import pytest
 
 from oracle import mjcf
 from oracle.pyoracle import OracleSim, cr_sincos
-from pippin_amd.physics import HipPhysics, selftest_math
+from pippin_amd.physics import HipPhysics, load_model, selftest_math
 from tests.scenes import GENERAL_SCENES, MULTIBODY_SCENES, PRIMITIVE_SCENES, general_xml, primitive_xml
 
 pytestmark = pytest.mark.gpu
@@ -295,7 +295,7 @@ def test_gripper_match():
     islands joined by contact."""
     from tests.scenes import GRIPPER
 
-    arr = mjcf.load_string(GRIPPER)
+    arr = load_model(GRIPPER, is_xml=True)  # the library's own loader
     n = 32
     o, g = OracleSim(arr, n), HipPhysics(arr, n)
     rng = np.random.default_rng(31)
@@ -322,7 +322,7 @@ def test_random_piles_match(seed, solver):
     """Random primitive piles with per-env perturbed drops, Newton and PGS."""
     from tests.scenes import random_pile_xml
 
-    arr = mjcf.load_string(random_pile_xml(seed))
+    arr = load_model(random_pile_xml(seed), is_xml=True)  # the library's own loader
     arr["solver_kind"] = 0 if solver == "newton" else 1
     n = 24
     o, g = OracleSim(arr, n), HipPhysics(arr, n)
@@ -373,7 +373,7 @@ def test_random_piles_with_general_pairs_match(seed, solver):
     """Random piles whose cylinders meet every other body."""
     from pippin_amd.workloads import random_pile_xml
 
-    arr = mjcf.load_string(random_pile_xml(seed, general=True))
+    arr = load_model(random_pile_xml(seed, general=True), is_xml=True)  # the library's own loader
     arr["solver_kind"] = 0 if solver == "newton" else 1
     n = 24
     o, g = OracleSim(arr, n), HipPhysics(arr, n)
@@ -399,7 +399,7 @@ def test_arm_gripper_with_per_env_cubes_match(solver):
     smooth random targets; Newton and PGS."""
     from pippin_amd.workloads import arm_gripper_xml, arm_targets
 
-    arr = mjcf.load_string(arm_gripper_xml(5))
+    arr = load_model(arm_gripper_xml(5), is_xml=True)  # the library's own loader
     arr["solver_kind"] = solver
     n = 24
     o, g = OracleSim(arr, n), HipPhysics(arr, n)

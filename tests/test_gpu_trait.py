@@ -261,3 +261,34 @@ def test_contact_capacity_overflow_fails_loudly_and_freezes_the_env(assets):
     with pytest.raises(HipError):
         g.step(0, 1, 1)  # the failed env reports again and stays put
     assert np.array_equal(g.get("qpos")[0], frozen[0])
+    # Data::reset brings the failed env back: it steps again from the reset state
+    ok.reset([0])
+    g.reset([0])
+    q_reset = g.get("qpos")[0].copy()
+    assert np.array_equal(q_reset, ok.get("qpos")[0])
+    g.step(0, 1, 2)  # no error: the env is no longer frozen
+    assert np.all(np.isfinite(g.get("qpos")[0]))
+    assert not np.array_equal(g.get("qpos")[0], q_reset)  # and it moves (the drop starts)
+
+
+def test_set_device_is_ordered_before_get_set_and_step(assets):
+    """set_device queues a device-to-device copy on the library stream;
+    get and set wait for it, and a host set after it wins."""
+    import torch
+
+    arr = mjcf.load_file(assets / "ant.xml")
+    n = 64
+    g, ref = HipPhysics(arr, n), OracleSim(arr, n)
+    rng = np.random.default_rng(5)
+    c_dev = rng.uniform(-1, 1, size=(n, 8))
+    buf = torch.from_numpy(c_dev).to("cuda")
+    g.set_device("ctrl", 0, n, buf.data_ptr())
+    assert np.array_equal(g.get("ctrl"), c_dev)
+    g.set_device("ctrl", 0, n, buf.data_ptr())
+    c_host = rng.uniform(-1, 1, size=(n, 8))
+    g.set("ctrl", 0, n, c_host)  # issued after the device copy: it must win
+    assert np.array_equal(g.get("ctrl"), c_host)
+    ref.set("ctrl", 0, n, c_host)
+    for s in (g, ref):
+        s.step(0, n, 3)
+    assert np.array_equal(g.get("qpos"), ref.get("qpos"))
