@@ -142,6 +142,21 @@ typedef struct ph_model_desc {
   const int32_t* exclude_pairs;       /* nexclude x 2 */
   const double* qpos0;                /* nq: kinematic reference */
   const double* qpos_reset;           /* nq: state Data::reset returns to */
+  /* Convex hulls of the mesh assets (Model::mesh_hull, model.rs:176-182, built by
+   * mesh::convex_hull, mesh.rs:63-69), indexed by geom_dataid. Per mesh: its hull
+   * vertices in the geom frame (scaled), sorted by (x, y, z), and its faces, each
+   * a convex polygon of hull-vertex indices (within the mesh) that runs
+   * counter-clockwise seen from outside and starts at its smallest index; faces
+   * sorted by their index lists. */
+  int32_t nmesh, nmeshvert, nmeshface, nmeshfacevert;
+  const int32_t* mesh_vertadr;        /* nmesh */
+  const int32_t* mesh_vertnum;        /* nmesh */
+  const int32_t* mesh_faceadr;        /* nmesh */
+  const int32_t* mesh_facenum;        /* nmesh */
+  const double* mesh_vert;            /* nmeshvert x 3 */
+  const int32_t* face_vertadr;        /* nmeshface: first entry in face_vert */
+  const int32_t* face_vertnum;        /* nmeshface */
+  const int32_t* face_vert;           /* nmeshfacevert */
 } ph_model_desc;
 
 /* World pose, crates/pippin-env/src/lib.rs:102-107 (#[repr(C)]). */

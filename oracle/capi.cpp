@@ -428,6 +428,7 @@ int orc_collide(int ta, const double* pa, const double* ma, const double* sa, in
   GeomPose a, b;
   a.typ = ta;
   b.typ = tb;
+  a.hull = b.hull = nullptr;
   for (int k = 0; k < 3; k++) {
     a.pos[k] = pa[k];
     a.size[k] = sa[k];

@@ -46,7 +46,18 @@ struct MomentEntry {
   Real coef;
 };
 
-// model.rs:102-249 (names, meshes and keyframes are not needed by the engine).
+// A mesh's convex hull as the narrow phase reads it (Model::mesh_hull,
+// model.rs:176-182, and the support queries of fasthull.rs): vertices in the
+// geom frame, faces (vertex lists, counter-clockwise from outside), the unit
+// normal of each face, and the faces around each vertex in increasing order.
+struct HullShape {
+  std::vector<Vec3> vert;
+  std::vector<std::vector<size_t>> faces;
+  std::vector<Vec3> normal;
+  std::vector<std::vector<size_t>> vert_faces;
+};
+
+// model.rs:102-249 (names and keyframes are not needed by the engine).
 struct Model {
   Real timestep = 0.002;
   Vec3 gravity = {{0, 0, -9.81}};
@@ -78,6 +89,7 @@ struct Model {
   std::vector<uint32_t> geom_contype, geom_conaffinity;
   std::vector<float> geom_rgba;  // ngeom x 4
   std::vector<std::pair<Vec3, Vec3>> geom_aabb;
+  std::vector<HullShape> mesh_hull;
 
   std::vector<size_t> cam_body;
   std::vector<Vec3> cam_pos;
@@ -122,6 +134,7 @@ struct GeomPose {
   Vec3 pos;
   Mat3 mat;
   Vec3 size;
+  const HullShape* hull;  // mesh geoms only
 };
 
 // newton.rs:34-49

@@ -215,8 +215,9 @@ void collide(const Model& m, Data& d) {
     for (int k = 0; k < 3 && !skip; k++)
       if (r_fabs(c1[k] - c2[k]) > h1[k] + h2[k] + margin) skip = true;
     if (skip) continue;
-    GeomPose p1 = {m.geom_type[g1], d.geom_xpos[g1], d.geom_xmat[g1], m.geom_size[g1]};
-    GeomPose p2 = {m.geom_type[g2], d.geom_xpos[g2], d.geom_xmat[g2], m.geom_size[g2]};
+    auto hull_of = [&](size_t g) { return m.geom_type[g] == Mesh ? &m.mesh_hull[m.geom_dataid[g]] : nullptr; };
+    GeomPose p1 = {m.geom_type[g1], d.geom_xpos[g1], d.geom_xmat[g1], m.geom_size[g1], hull_of(g1)};
+    GeomPose p2 = {m.geom_type[g2], d.geom_xpos[g2], d.geom_xmat[g2], m.geom_size[g2], hull_of(g2)};
     hits.clear();
     if (needs_general(m.geom_type[g1], m.geom_type[g2])) collide_general(p1, p2, margin, hits);
     else collide_pair(p1, p2, margin, hits);

@@ -2,8 +2,8 @@
 //
 // General convex narrow phase: the pairs that crates/pippin/src/collision.rs:102-107
 // (needs_general) sends to parry3d-f64 0.31.1 through collide_general
-// (collision.rs:124-147). Supported here: a cylinder against a sphere, capsule, box
-// or cylinder. Mesh geoms are not loaded by this oracle.
+// (collision.rs:124-147): a cylinder against a sphere, capsule, box or cylinder,
+// and a mesh (its convex hull) against a plane or any other geom.
 //
 // parry is a crates.io dependency that is not in the reference tree, so this is a
 // restatement of its published method, not of its code:
@@ -13,7 +13,9 @@
 //   3. the support feature of each shape along the contact normal, as parry's
 //      PolygonalFeatureMap gives it (box: the face most aligned; cylinder: the cap
 //      approximated by a square when |n_axis| >= 0.5, else the side segment;
-//      capsule: its segment; sphere: its centre), and their overlap in the plane
+//      capsule: its segment; sphere: its centre; hull: the face most aligned
+//      among those around the support vertex, at most its first four vertices,
+//      as FastHull::local_support_feature, fasthull.rs:125-153), and their overlap in the plane
 //      normal to the contact normal (polygon clip, segment clip, parallel
 //      segments), each point lifted back onto both features;
 //   4. points whose separation is below the margin become hits; more than four
@@ -22,6 +24,13 @@
 // The hit is the midpoint of the two surface points, the normal points from the
 // first geom to the second and depth = -separation, as collide_general converts
 // parry's manifold (collision.rs:139-146).
+// A plane against a hull is parry's half-space manifold: every vertex of the
+// hull's support feature along the plane's inward normal whose distance to the
+// plane is at most the margin is a hit, at the midpoint between the vertex and
+// its projection on the plane.
+// A hull's support point is the first of its vertices with the largest
+// projection (a scan: FastHull scans hulls under 64 vertices, fasthull.rs:85-96;
+// larger ones hill-climb to the same maximum, ties aside).
 //
 // Parity against parry is unpinned (its arithmetic is not available here). The
 // tests pin this restatement with the reference's own physical tests of the path
@@ -50,6 +59,7 @@ struct Core {
   Mat3 mat;
   Vec3 size;
   Real rad;
+  const HullShape* hull;
 };
 
 Core core_of(const GeomPose& g) {
@@ -59,11 +69,27 @@ Core core_of(const GeomPose& g) {
   c.mat = g.mat;
   c.size = g.size;
   c.rad = (g.typ == Sphere || g.typ == Capsule) ? g.size[0] : Real(0.0);
+  c.hull = g.hull;
   return c;
+}
+
+// Index of the first hull vertex with the largest projection on d (local frame).
+size_t hull_support(const HullShape& h, Vec3 d) {
+  size_t v = 0;
+  Real best = -INFINITY;
+  for (size_t i = 0; i < h.vert.size(); i++) {
+    Real x = dot(h.vert[i], d);
+    if (x > best) {
+      best = x;
+      v = i;
+    }
+  }
+  return v;
 }
 
 // Extreme point of the core along d, local frame.
 Vec3 support_local(const Core& c, Vec3 d) {
+  if (c.typ == Mesh) return c.hull->vert[hull_support(*c.hull, d)];
   if (c.typ == Sphere) return v3zero();
   if (c.typ == Capsule) return V3(0.0, 0.0, d[2] >= 0.0 ? c.size[1] : -c.size[1]);
   if (c.typ == Box)
@@ -365,7 +391,23 @@ int feature(const Core& c, Vec3 dir, Vec3* out) {
   Vec3 d = tmul_vec(c.mat, dir);
   Vec3 loc[4];
   int n;
-  if (c.typ == Sphere) {
+  if (c.typ == Mesh) {
+    // the face most aligned with d among those around the support vertex
+    // (Iterator::max_by: the last of equal maxima)
+    const HullShape& h = *c.hull;
+    size_t best = h.vert_faces[hull_support(h, d)][0];
+    Real bd = dot(h.normal[best], d);
+    for (size_t f : h.vert_faces[hull_support(h, d)]) {
+      Real x = dot(h.normal[f], d);
+      if (x >= bd) {
+        bd = x;
+        best = f;
+      }
+    }
+    const std::vector<size_t>& face = h.faces[best];
+    n = face.size() < 4 ? (int)face.size() : 4;
+    for (int q = 0; q < n; q++) loc[q] = h.vert[face[(size_t)q]];
+  } else if (c.typ == Sphere) {
     loc[0] = v3zero();
     n = 1;
   } else if (c.typ == Capsule) {
@@ -564,11 +606,39 @@ void poly_poly(const Vec3* a, int na, const Vec3* b, int nb, Vec3 n, Vec3 t1, Ve
   }
 }
 
+// A plane against a hull: the hull's support feature along the plane's inward
+// normal, each vertex within the margin a hit. `plane_first`: the plane is the
+// pair's first geom (normal along the plane's normal), else its negation.
+void plane_hull(const GeomPose& pl, const Core& H, bool plane_first, Real margin, std::vector<Hit>& out) {
+  Vec3 n = col(pl.mat, 2);
+  Vec3 fv[4];
+  int nv = feature(H, -n, fv);
+  for (int k = 0; k < nv; k++) {
+    Real dist = dot(fv[k] - pl.pos, n);
+    if (dist <= margin) {
+      Hit h;
+      h.pos = fv[k] - n * (dist * 0.5);
+      h.normal = plane_first ? n : -n;
+      h.depth = -dist;
+      h.tangent = v3zero();
+      out.push_back(h);
+    }
+  }
+}
+
 }  // namespace
 
 // collision.rs:124-147 (parry contact manifold), restated as described above.
 void collide_general(const GeomPose& ga, const GeomPose& gb, Real margin, std::vector<Hit>& out) {
-  if (ga.typ == Mesh || gb.typ == Mesh || ga.typ == Plane || gb.typ == Plane) return;
+  if (ga.typ == Plane && gb.typ == Mesh) {
+    plane_hull(ga, core_of(gb), true, margin, out);
+    return;
+  }
+  if (gb.typ == Plane && ga.typ == Mesh) {
+    plane_hull(gb, core_of(ga), false, margin, out);
+    return;
+  }
+  if (ga.typ == Plane || gb.typ == Plane) return;
   Core A = core_of(ga), B = core_of(gb);
   Simplex s;
   Vec3 d0 = B.pos - A.pos;
@@ -634,9 +704,9 @@ void collide_general(const GeomPose& ga, const GeomPose& gb, Real margin, std::v
   Cands c;
   c.n = 0;
   if (na == 2 && nb == 2) seg_seg(fa, fb, n, t1, t2, A.rad, B.rad, margin, c);
-  else if (na == 2 && nb == 4) seg_poly(fa, fb, nb, true, n, t1, t2, A.rad, B.rad, margin, c);
-  else if (na == 4 && nb == 2) seg_poly(fb, fa, na, false, n, t1, t2, A.rad, B.rad, margin, c);
-  else if (na == 4 && nb == 4) poly_poly(fa, na, fb, nb, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na == 2 && nb >= 3) seg_poly(fa, fb, nb, true, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na >= 3 && nb == 2) seg_poly(fb, fa, na, false, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na >= 3 && nb >= 3) poly_poly(fa, na, fb, nb, n, t1, t2, A.rad, B.rad, margin, c);
   if (c.n == 0) emit(c, wa, wb, n, A.rad, B.rad, margin);
   std::vector<std::pair<Real, Vec3>> pts;
   for (int k = 0; k < c.n; k++) pts.push_back({c.dist[k], c.pos[k]});

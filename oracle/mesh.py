@@ -40,6 +40,27 @@ def read_obj_vertices(text: str, scale=(1.0, 1.0, 1.0)):
     return np.array(out, dtype=np.float64).reshape(-1, 3)
 
 
+def read_stl_vertices(data: bytes, scale=(1.0, 1.0, 1.0)):
+    """Triangle corners of a binary STL (50 bytes a triangle after an 84-byte
+    header) or an ASCII one ('vertex x y z'), as float32 (stl_io's precision),
+    widened and scaled per axis."""
+    out = []
+    if len(data) >= 84:
+        n = struct.unpack_from("<I", data, 80)[0]
+        if len(data) == 84 + 50 * n:
+            for t in range(n):
+                for v in range(3):
+                    xyz = struct.unpack_from("<3f", data, 84 + 50 * t + 12 + 12 * v)
+                    out.append([xyz[0] * scale[0], xyz[1] * scale[1], xyz[2] * scale[2]])
+            return np.array(out, dtype=np.float64).reshape(-1, 3)
+    toks = data.decode(errors="replace").split()
+    for i, tk in enumerate(toks):
+        if tk == "vertex":
+            xyz = [struct.unpack("f", struct.pack("f", float(t)))[0] for t in toks[i + 1:i + 4]]
+            out.append([xyz[0] * scale[0], xyz[1] * scale[1], xyz[2] * scale[2]])
+    return np.array(out, dtype=np.float64).reshape(-1, 3)
+
+
 def _hull2d(pts2, idx):
     """Monotone chain: indices of the strict convex hull of 2-D points,
     counter-clockwise, collinear points dropped."""

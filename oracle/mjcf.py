@@ -20,6 +20,8 @@ from pathlib import Path
 
 import numpy as np
 
+from . import mesh as mesh_mod
+
 PI = math.pi
 JNT = {"free": 0, "ball": 1, "slide": 2, "hinge": 3}
 GEOM = {"plane": 0, "sphere": 1, "capsule": 2, "box": 3, "cylinder": 4, "mesh": 5}
@@ -261,6 +263,7 @@ def geom_volume_inertia(gt, s):
 
 class _Builder:
     def __init__(self):
+        self.meshes = []  # (name, hull vertices, hull faces)
         self.nq = self.nv = 0
         self.timestep = 0.002
         self.gravity = (0.0, 0.0, -9.81)
@@ -408,8 +411,15 @@ def _parse_geom(ctx, bld, node, body, cls):
     if tname not in GEOM:
         return  # unsupported geom types are skipped with a warning
     gt = GEOM[tname]
+    dataid = -1
     if gt == 5:
-        raise MjcfError("mesh geoms need the convex-hull pipeline, which this loader does not restate")
+        mname = ctx.get(node, cls, "mesh")
+        if mname is None:
+            raise MjcfError("mesh geom needs a 'mesh' attribute")
+        names = [mm[0] for mm in bld.meshes]
+        if mname not in names:
+            raise MjcfError(f"unknown mesh '{mname}'")
+        dataid = names.index(mname)
     sz = ctx.floats(node, cls, "size") or []
     get = lambda i: sz[i] if i < len(sz) else 0.0
     size = (get(0), get(1), get(2))
@@ -441,7 +451,23 @@ def _parse_geom(ctx, bld, node, body, cls):
         mat = ctx.get(node, cls, "material")
         rgba = ctx.materials.get(mat) if mat is not None else None
     group = _as_i32(ctx.float(node, cls, "group", 0.0))
-    if gt != 0:
+    if gt == 5:  # mjcf.rs:686-703: the hull's solid mass properties
+        _name, hv, hf = bld.meshes[dataid]
+        vol, com, icom = mesh_mod.mass_properties(hv, mesh_mod.triangles(hf))
+        mv = ctx.floats(node, cls, "mass")
+        mass = mv[0] if mv is not None else ctx.float(node, cls, "density", 1000.0) * vol
+        if mass > 0.0 and vol > 0.0:
+            r = to_mat(quat)
+            rc = tuple(r[3 * i] * com[0] + r[3 * i + 1] * com[1] + r[3 * i + 2] * com[2] for i in range(3))
+            c = vadd(pos, rc)
+            ig = mrotate(mscale(tuple(float(x) for x in np.asarray(icom).reshape(9)), mass / vol), r)
+            cc = mskew(c)
+            shifted = madd(ig, mscale(mmul(cc, mT(cc)), mass))
+            e = bld.geom_mass[body]
+            e[0] += mass
+            e[1] = vadd(e[1], vscale(c, mass))
+            e[2] = madd(e[2], shifted)
+    elif gt != 0:
         vol, unit = geom_volume_inertia(gt, size)
         mv = ctx.floats(node, cls, "mass")
         mass = mv[0] if mv is not None else ctx.float(node, cls, "density", 1000.0) * vol
@@ -464,7 +490,7 @@ def _parse_geom(ctx, bld, node, body, cls):
     g["conaffinity"].append(conaffinity)
     g["rgba"].append(rgba if rgba is not None else (0.5, 0.5, 0.5, 1.0))
     g["group"].append(group)
-    g["dataid"].append(-1)
+    g["dataid"].append(dataid)
 
 
 def _as_u32(x):
@@ -650,13 +676,46 @@ def _parse_equality(ctx, bld, node):
     e["solimp"].append(solimp)
 
 
+def _parse_mesh(ctx, bld, node, meshdir):
+    """mjcf.rs:740-772: vertices from an OBJ/STL file or the 'vertex' attribute,
+    scaled per axis, then their convex hull (a degenerate set is an error)."""
+    f = node.get("file")
+    name = node.get("name") or (Path(f).stem if f else None)
+    if name is None:
+        raise MjcfError("mesh needs a name or file")
+    scale = ctx.vec3(node, "main", "scale", (1.0, 1.0, 1.0))
+    if f is not None:
+        path = meshdir / f
+        ext = path.suffix.lower()
+        if ext == ".obj":
+            pts = mesh_mod.read_obj_vertices(path.read_text(), scale)
+        elif ext == ".stl":
+            pts = mesh_mod.read_stl_vertices(path.read_bytes(), scale)
+        else:
+            raise MjcfError(f"unsupported mesh format '{ext[1:]}' (use .stl or .obj)")
+    else:
+        v = ctx.floats(node, "main", "vertex")
+        if v is None:
+            raise MjcfError(f"mesh '{name}' needs 'file' or 'vertex'")
+        pts = np.array([(v[i] * scale[0], v[i + 1] * scale[1], v[i + 2] * scale[2])
+                        for i in range(0, len(v) - 2, 3)], dtype=np.float64).reshape(-1, 3)
+    hull = mesh_mod.convex_hull(pts)
+    if hull is None:
+        raise MjcfError(f"mesh '{name}' is degenerate (no volume)")
+    bld.meshes.append((name, hull[0], hull[1]))
+
+
 def load_string(xml: str, directory: str | Path = ".") -> dict:
-    """mjcf.rs:211-395 (without <include> and meshes)."""
+    """mjcf.rs:211-395 (without <include>)."""
     root = ET.fromstring(xml)
     if root.tag != "mujoco":
         raise MjcfError("root element must be <mujoco>")
     ctx = Ctx()
+    meshdir = Path(directory)
     for c in root.findall("compiler"):
+        d = c.get("meshdir", c.get("assetdir"))
+        if d is not None:
+            meshdir = Path(directory) / d
         a = c.get("angle")
         if a is not None:
             ctx.degrees = a == "degree"
@@ -700,8 +759,8 @@ def load_string(xml: str, directory: str | Path = ".") -> dict:
         if v is not None:
             bld.integrator = 1 if v in ("implicitfast", "implicit") else 0
     for a in root.findall("asset"):
-        if a.findall("mesh"):
-            raise MjcfError("mesh assets need the convex-hull pipeline, which this loader does not restate")
+        for mn in a.findall("mesh"):
+            _parse_mesh(ctx, bld, mn, meshdir)
         for mat in a.findall("material"):
             cls = mat.get("class", "main")
             name = mat.get("name")
@@ -742,6 +801,26 @@ def load_string(xml: str, directory: str | Path = ".") -> dict:
 def load_file(path: str | Path) -> dict:
     path = Path(path)
     return load_string(path.read_text(), path.parent)
+
+
+def _mesh_arrays(meshes) -> dict:
+    """The descriptor's hull arrays: per mesh its vertices and faces, flattened."""
+    vadr, vnum, fadr, fnum, verts, fvadr, fvnum, fvert = [], [], [], [], [], [], [], []
+    for _name, V, faces in meshes:
+        vadr.append(len(verts))
+        vnum.append(len(V))
+        verts.extend(np.asarray(V, dtype=np.float64).reshape(-1, 3).tolist())
+        fadr.append(len(fvadr))
+        fnum.append(len(faces))
+        for f in faces:
+            fvadr.append(len(fvert))
+            fvnum.append(len(f))
+            fvert.extend(int(t) for t in f)
+    i32 = lambda x: np.array(x, dtype=np.int32)
+    return dict(nmesh=len(meshes), nmeshvert=len(verts), nmeshface=len(fvadr), nmeshfacevert=len(fvert),
+                mesh_vertadr=i32(vadr), mesh_vertnum=i32(vnum), mesh_faceadr=i32(fadr), mesh_facenum=i32(fnum),
+                mesh_vert=np.array(verts, dtype=np.float64).reshape(-1, 3), face_vertadr=i32(fvadr),
+                face_vertnum=i32(fvnum), face_vert=i32(fvert))
 
 
 def _to_arrays(bld: _Builder) -> dict:
@@ -796,4 +875,5 @@ def _to_arrays(bld: _Builder) -> dict:
         eq_solref=f64(e["solref"], 2), eq_solimp=f64(e["solimp"], 5),
         exclude_pairs=np.array(bld.exclude, dtype=np.int32).reshape(-1, 2),
         qpos0=np.array(bld.qpos0, dtype=np.float64), qpos_reset=np.array(bld.qpos0, dtype=np.float64),
+        **_mesh_arrays(bld.meshes),
     )

@@ -49,7 +49,7 @@ static std::vector<Range> compute_dof_blocks(const Model& m) {
   return blocks;
 }
 
-// model.rs:366-382 (mesh hulls are not supported by this oracle)
+// model.rs:366-382
 std::pair<Vec3, Vec3> local_aabb(const Model& m, size_t g) {
   Vec3 s = m.geom_size[g];
   Real inf = INFINITY;
@@ -59,6 +59,16 @@ std::pair<Vec3, Vec3> local_aabb(const Model& m, size_t g) {
     case Capsule: return {v3zero(), V3(s[0], s[0], s[0] + s[1])};
     case Box: return {v3zero(), s};
     case Cylinder: return {v3zero(), V3(s[0], s[0], s[1])};
+    case Mesh: {
+      const HullShape& h = m.mesh_hull[m.geom_dataid[g]];
+      Vec3 lo = V3(inf, inf, inf), hi = V3(-inf, -inf, -inf);
+      for (const Vec3& p : h.vert)
+        for (int k = 0; k < 3; k++) {
+          lo[k] = p[k] < lo[k] ? p[k] : lo[k];
+          hi[k] = p[k] > hi[k] ? p[k] : hi[k];
+        }
+      return {(lo + hi) * 0.5, (hi - lo) * 0.5};
+    }
     default: return {v3zero(), v3zero()};
   }
 }
@@ -72,8 +82,37 @@ Real rbound(const Model& m, size_t g) {
     case Capsule: return s[0] + s[1];
     case Box: return norm(s);
     case Cylinder: return r_sqrt(s[0] * s[0] + s[1] * s[1]);
+    case Mesh: {
+      Real r = 0.0;
+      for (const Vec3& p : m.mesh_hull[m.geom_dataid[g]].vert) {
+        Real l = norm(p);
+        r = l > r ? l : r;
+      }
+      return r;
+    }
     default: return 0.0;
   }
+}
+
+// A hull from the descriptor's mesh arrays, with its face normals
+// (cross(v1 - v0, v2 - v0) over its length) and the faces around each vertex.
+static HullShape hull_from_desc(const ph_model_desc& d, int mesh) {
+  HullShape h;
+  const int va = d.mesh_vertadr[mesh], vn = d.mesh_vertnum[mesh];
+  for (int i = 0; i < vn; i++)
+    h.vert.push_back(V3(d.mesh_vert[3 * (va + i)], d.mesh_vert[3 * (va + i) + 1], d.mesh_vert[3 * (va + i) + 2]));
+  h.vert_faces.resize((size_t)vn);
+  const int fa = d.mesh_faceadr[mesh], fn = d.mesh_facenum[mesh];
+  for (int f = 0; f < fn; f++) {
+    std::vector<size_t> face;
+    for (int k = 0; k < d.face_vertnum[fa + f]; k++) face.push_back((size_t)d.face_vert[d.face_vertadr[fa + f] + k]);
+    for (size_t v : face) h.vert_faces[v].push_back((size_t)f);
+    Vec3 nrm = cross(h.vert[face[1]] - h.vert[face[0]], h.vert[face[2]] - h.vert[face[0]]);
+    Real l = r_sqrt(dot(nrm, nrm));
+    h.normal.push_back(V3(nrm[0] / l, nrm[1] / l, nrm[2] / l));
+    h.faces.push_back(face);
+  }
+  return h;
 }
 
 // model.rs:430-452
@@ -184,6 +223,7 @@ Model model_from_desc(const ph_model_desc& d) {
     m.dof_damping.push_back(d.dof_damping[k]);
     m.dof_armature.push_back(d.dof_armature[k]);
   }
+  for (int k = 0; k < d.nmesh; k++) m.mesh_hull.push_back(hull_from_desc(d, k));
   for (int g = 0; g < d.ngeom; g++) {
     m.geom_type.push_back(d.geom_type[g]);
     m.geom_body.push_back((size_t)d.geom_body[g]);

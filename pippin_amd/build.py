@@ -14,7 +14,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libpippin_hip.so"
-SOURCES = ["runtime.hip", "mlp.hip", "host_model.cpp", "mjcf.cpp"]
+SOURCES = ["runtime.hip", "mlp.hip", "host_model.cpp", "mjcf.cpp", "meshes.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PIPPIN_HIP_ARCH", "gfx950")
 FLAGS = [
@@ -32,7 +32,7 @@ SPECIALISED = [("SpecAnt", HERE / "assets" / "ant.xml")]
 def generate_specs(bdir: Path, verbose: bool = False) -> None:
     """Build the host-only generator and write _build/spec_models.hpp."""
     gen = bdir / "spec_gen"
-    srcs = [CSRC / "spec_gen.cpp", CSRC / "host_model.cpp", CSRC / "mjcf.cpp"]
+    srcs = [CSRC / "spec_gen.cpp", CSRC / "host_model.cpp", CSRC / "mjcf.cpp", CSRC / "meshes.cpp"]
     cmd = [HIPCC, "-O1", "-std=c++17", "-x", "hip", f"--offload-arch={ARCH}", *map(str, srcs), "-o", str(gen)]
     if verbose:
         print(" ".join(cmd), flush=True)

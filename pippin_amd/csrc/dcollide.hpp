@@ -7,11 +7,22 @@
 
 namespace phk {
 
+// A mesh geom's convex hull as the general narrow phase reads it.
+struct HullRef {
+  gp<const Real> vert;          // this hull's vertices (geom frame), 3 per vertex
+  gp<const int> vfadr, vfnum;   // per vertex of this hull: its faces in vface
+  gp<const int> vface;          // face ids (all hulls), increasing per vertex
+  gp<const Real> fnormal;       // unit face normals (all hulls), 3 per face
+  gp<const int> fvadr, fvnum, fvert;  // face vertex lists, ids within the face's hull
+  int nvert;
+};
+
 struct GPose {
   int typ;
   V3 pos;
   M3 mat;
   V3 size;
+  HullRef hull;  // mesh geoms only
 };
 
 struct Sink {

@@ -1,7 +1,8 @@
 // General convex narrow phase on the device: the pairs that
 // crates/pippin/src/collision.rs:102-107 (needs_general) sends to parry3d-f64
-// 0.31.1 (collide_general, collision.rs:124-147). Supported: a cylinder against a
-// sphere, capsule, box or cylinder.
+// 0.31.1 (collide_general, collision.rs:124-147): a cylinder against a sphere,
+// capsule, box or cylinder, and a mesh (its convex hull) against a plane or any
+// other geom.
 //
 // parry's published method, written for one lane per pair with fixed-size
 // private arrays:
@@ -11,15 +12,21 @@
 //      faces, 28 expansions, stop when the support gains <= 1e-9);
 //   3. the support feature of each shape along the normal (box: most aligned
 //      face; cylinder: cap as a square when |n_axis| >= 0.5, else the side
-//      segment; capsule: its segment; sphere: its centre), overlapped in the
-//      plane normal to the contact normal, each point lifted back onto both
-//      features;
+//      segment; capsule: its segment; sphere: its centre; hull: the face most
+//      aligned among those around the support vertex, at most its first four
+//      vertices, fasthull.rs:125-153), overlapped in the plane normal to the
+//      contact normal, each point lifted back onto both features;
 //   4. points with separation below the margin are hits, more than four are
 //      reduced with the box-box rule (collision.rs:514-531), and the GJK/EPA
 //      witness pair is the hit when the features give none.
 // Hits: midpoint of the two surface points, normal from the first geom to the
-// second, depth = -separation (collision.rs:139-146). Every operation is in the
-// order of the CPU restatement that the tests compare against, bit for bit.
+// second, depth = -separation (collision.rs:139-146). A plane against a hull
+// takes the hull's support feature along the plane's inward normal: each of its
+// vertices within the margin is a hit, midway between it and the plane. A hull's
+// support point is the first of its vertices with the largest projection (a
+// scan, as FastHull does below 64 vertices, fasthull.rs:85-96). Every operation
+// is in the order of the CPU restatement that the tests compare against, bit
+// for bit.
 #pragma once
 
 #include "dcollide.hpp"
@@ -41,6 +48,7 @@ struct Core {
   M3 mat;
   V3 size;
   Real rad;
+  HullRef hull;
 };
 
 PHD Real vget(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -57,10 +65,28 @@ PHD Core core_of(const GPose& g) {
   c.mat = g.mat;
   c.size = g.size;
   c.rad = (g.typ == G_SPHERE || g.typ == G_CAPSULE) ? g.size.x : 0.0;
+  c.hull = g.hull;
   return c;
 }
 
+PHD V3 hull_vert(const HullRef& h, int i) { return v3(h.vert[3 * i], h.vert[3 * i + 1], h.vert[3 * i + 2]); }
+
+// First hull vertex with the largest projection on d (local frame).
+PHD int hull_support(const HullRef& h, V3 d) {
+  int v = 0;
+  Real best = -INFINITY;
+  for (int i = 0; i < h.nvert; i++) {
+    const Real x = dot(hull_vert(h, i), d);
+    if (x > best) {
+      best = x;
+      v = i;
+    }
+  }
+  return v;
+}
+
 PHD V3 support_local(const Core& c, V3 d) {
+  if (c.typ == G_MESH) return hull_vert(c.hull, hull_support(c.hull, d));
   if (c.typ == G_SPHERE) return v3z();
   if (c.typ == G_CAPSULE) return v3(0.0, 0.0, d.z >= 0.0 ? c.size.y : -c.size.y);
   if (c.typ == G_BOX)
@@ -358,7 +384,26 @@ PHD int feature(const Core& c, V3 dir, V3* out) {
   V3 d = tmul_vec(c.mat, dir);
   V3 loc[4];
   int n;
-  if (c.typ == G_SPHERE) {
+  if (c.typ == G_MESH) {
+    // the face most aligned with d among those around the support vertex
+    // (Iterator::max_by: the last of equal maxima)
+    const HullRef& h = c.hull;
+    const int v = hull_support(h, d);
+    const int fa = h.vfadr[v], fn = h.vfnum[v];
+    int best = h.vface[fa];
+    Real bd = dot(v3(h.fnormal[3 * best], h.fnormal[3 * best + 1], h.fnormal[3 * best + 2]), d);
+    for (int i = 0; i < fn; i++) {
+      const int f = h.vface[fa + i];
+      const Real x = dot(v3(h.fnormal[3 * f], h.fnormal[3 * f + 1], h.fnormal[3 * f + 2]), d);
+      if (x >= bd) {
+        bd = x;
+        best = f;
+      }
+    }
+    const int k = h.fvnum[best];
+    n = k < 4 ? k : 4;
+    for (int q = 0; q < n; q++) loc[q] = hull_vert(h, h.fvert[h.fvadr[best] + q]);
+  } else if (c.typ == G_SPHERE) {
     loc[0] = v3z();
     n = 1;
   } else if (c.typ == G_CAPSULE) {
@@ -609,6 +654,19 @@ PHD void reduce4(Cands& c) {
   c.n = 4;
 }
 
+// A plane against a hull: the hull's support feature along the plane's inward
+// normal, each vertex within the margin a hit. plane_first: the plane is the
+// pair's first geom (normal along the plane's), else the normal is negated.
+PHD void plane_hull(const GPose& pl, const Core& H, bool plane_first, Real margin, Sink& out) {
+  const V3 n = col(pl.mat, 2);
+  V3 fv[4];
+  const int nv = feature(H, -n, fv);
+  for (int k = 0; k < nv; k++) {
+    const Real dist = dot(fv[k] - pl.pos, n);
+    if (dist <= margin) out.push(fv[k] - n * (dist * 0.5), plane_first ? n : -n, -dist);
+  }
+}
+
 }  // namespace gen
 
 PHD bool needs_general(int a, int b) {
@@ -618,7 +676,15 @@ PHD bool needs_general(int a, int b) {
 // collision.rs:124-147, the method described at the top of this file.
 inline __host__ __device__ __noinline__ void collide_general(const GPose& ga, const GPose& gb, Real margin, Sink& out) {
   using namespace gen;
-  if (ga.typ == G_MESH || gb.typ == G_MESH || ga.typ == G_PLANE || gb.typ == G_PLANE) return;
+  if (ga.typ == G_PLANE && gb.typ == G_MESH) {
+    plane_hull(ga, core_of(gb), true, margin, out);
+    return;
+  }
+  if (gb.typ == G_PLANE && ga.typ == G_MESH) {
+    plane_hull(gb, core_of(ga), false, margin, out);
+    return;
+  }
+  if (ga.typ == G_PLANE || gb.typ == G_PLANE) return;
   Core A = core_of(ga), B = core_of(gb);
   Simplex s;
   V3 d0 = B.pos - A.pos;
@@ -684,9 +750,9 @@ inline __host__ __device__ __noinline__ void collide_general(const GPose& ga, co
   Cands c;
   c.n = 0;
   if (na == 2 && nb == 2) seg_seg(fa, fb, n, t1, t2, A.rad, B.rad, margin, c);
-  else if (na == 2 && nb == 4) seg_poly(fa, fb, nb, true, n, t1, t2, A.rad, B.rad, margin, c);
-  else if (na == 4 && nb == 2) seg_poly(fb, fa, na, false, n, t1, t2, A.rad, B.rad, margin, c);
-  else if (na == 4 && nb == 4) poly_poly(fa, na, fb, nb, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na == 2 && nb >= 3) seg_poly(fa, fb, nb, true, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na >= 3 && nb == 2) seg_poly(fb, fa, na, false, n, t1, t2, A.rad, B.rad, margin, c);
+  else if (na >= 3 && nb >= 3) poly_poly(fa, na, fb, nb, n, t1, t2, A.rad, B.rad, margin, c);
   if (c.n == 0) emit(c, wa, wb, n, A.rad, B.rad, margin);
   reduce4(c);
   for (int k = 0; k < c.n; k++) out.push(c.pos[k], n, -c.dist[k]);

@@ -123,6 +123,13 @@ struct DModel {
   gp<const int> eq_j1, eq_j2;
   gp<const Real> eq_poly, eq_solref, eq_solimp;
   gp<const Real> qpos0;
+  // convex hulls of mesh geoms (geom_dataid: mesh index or -1)
+  gp<const int> geom_dataid;
+  gp<const int> mesh_vertadr, mesh_vertnum;  // per mesh: first hull vertex, count
+  gp<const Real> mesh_vert;                  // hull vertices, geom frame, 3 per vertex
+  gp<const Real> face_normal;                // unit normal per face, 3 per face
+  gp<const int> face_vertadr, face_vertnum, face_vert;  // face vertex lists (ids within the face's mesh)
+  gp<const int> vert_faceadr, vert_facenum, vert_face;  // per hull vertex: faces around it, increasing
 
   // per-variant parameters (params.rs), variant of env e = env_var[e]
   gp<const int> env_var;

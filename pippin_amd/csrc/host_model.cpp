@@ -74,6 +74,77 @@ void retarget(HostModel& m) {
   d.exclude_pairs = m.exclude_pairs.data();
   d.qpos0 = m.qpos0.data();
   d.qpos_reset = m.qpos_reset.data();
+  d.mesh_vertadr = m.mesh_vertadr.data();
+  d.mesh_vertnum = m.mesh_vertnum.data();
+  d.mesh_faceadr = m.mesh_faceadr.data();
+  d.mesh_facenum = m.mesh_facenum.data();
+  d.mesh_vert = m.mesh_vert.data();
+  d.face_vertadr = m.face_vertadr.data();
+  d.face_vertnum = m.face_vertnum.data();
+  d.face_vert = m.face_vert.data();
+}
+
+// Copy the hull arrays of `s`, check their indices, and derive face normals and
+// the faces around each vertex. False (err set) for malformed hulls.
+static bool import_hulls(const ph_model_desc& s, HostModel& m, std::string* err) {
+  if (s.nmesh < 0 || s.nmeshvert < 0 || s.nmeshface < 0 || s.nmeshfacevert < 0) {
+    *err = "negative mesh counts";
+    return false;
+  }
+  cp(m.mesh_vertadr, s.mesh_vertadr, s.nmesh);
+  cp(m.mesh_vertnum, s.mesh_vertnum, s.nmesh);
+  cp(m.mesh_faceadr, s.mesh_faceadr, s.nmesh);
+  cp(m.mesh_facenum, s.mesh_facenum, s.nmesh);
+  cp(m.mesh_vert, s.mesh_vert, 3 * (int64_t)s.nmeshvert);
+  cp(m.face_vertadr, s.face_vertadr, s.nmeshface);
+  cp(m.face_vertnum, s.face_vertnum, s.nmeshface);
+  cp(m.face_vert, s.face_vert, s.nmeshfacevert);
+  m.face_normal.assign(3 * (size_t)s.nmeshface, 0.0);
+  m.vert_faceadr.assign(s.nmeshvert, 0);
+  m.vert_facenum.assign(s.nmeshvert, 0);
+  std::vector<std::vector<int32_t>> around(s.nmeshvert);
+  for (int k = 0; k < s.nmesh; k++) {
+    const int va = m.mesh_vertadr[k], vn = m.mesh_vertnum[k], fa = m.mesh_faceadr[k], fn = m.mesh_facenum[k];
+    if (va < 0 || vn < 4 || va + vn > s.nmeshvert || vn > 64 || fa < 0 || fn < 4 || fa + fn > s.nmeshface) {
+      *err = "mesh " + std::to_string(k) + ": hull ranges out of bounds (or more than 64 vertices)";
+      return false;
+    }
+    for (int f = fa; f < fa + fn; f++) {
+      const int a = m.face_vertadr[f], c = m.face_vertnum[f];
+      if (a < 0 || c < 3 || a + c > s.nmeshfacevert) {
+        *err = "mesh " + std::to_string(k) + ": face ranges out of bounds";
+        return false;
+      }
+      for (int i = 0; i < c; i++) {
+        const int v = m.face_vert[a + i];
+        if (v < 0 || v >= vn) {
+          *err = "mesh " + std::to_string(k) + ": face vertex out of range";
+          return false;
+        }
+        around[va + v].push_back(f);
+      }
+      auto P = [&](int i) {
+        const double* p = &m.mesh_vert[3 * (size_t)(va + m.face_vert[a + i])];
+        return phk::v3(p[0], p[1], p[2]);
+      };
+      const phk::V3 n = phk::cross(P(1) - P(0), P(2) - P(0));
+      const double l = std::sqrt(phk::dot(n, n));
+      m.face_normal[3 * f] = n.x / l;
+      m.face_normal[3 * f + 1] = n.y / l;
+      m.face_normal[3 * f + 2] = n.z / l;
+    }
+  }
+  m.vert_face.clear();
+  for (int v = 0; v < s.nmeshvert; v++) {
+    m.vert_faceadr[v] = (int)m.vert_face.size();
+    m.vert_facenum[v] = (int)around[v].size();
+    if (around[v].empty()) {
+      *err = "hull vertex " + std::to_string(v) + " lies on no face";
+      return false;
+    }
+    m.vert_face.insert(m.vert_face.end(), around[v].begin(), around[v].end());
+  }
+  return true;
 }
 
 void local_aabb(int t, const double* s, double* o) {
@@ -244,6 +315,12 @@ bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
   cp(m.qpos0, s.qpos0, s.nq);
   cp(m.qpos_reset, s.qpos_reset, s.nq);
   if ((int)m.qpos_reset.size() != s.nq) m.qpos_reset = m.qpos0;
+  if (!import_hulls(s, m, err)) return false;
+  for (int g = 0; g < ng; g++)
+    if (m.geom_type[g] == PH_GEOM_MESH && (m.geom_dataid[g] < 0 || m.geom_dataid[g] >= s.nmesh)) {
+      *err = "mesh geom " + std::to_string(g) + " names no mesh";
+      return false;
+    }
   retarget(m);
 
   // dof tree (model.rs:285-296)
@@ -316,7 +393,23 @@ bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
   m.acc_child_adr.push_back((int)m.acc_child.size());
   // local boxes
   m.geom_aabb.assign(6 * ng, 0.0);
-  for (int g = 0; g < ng; g++) local_aabb(m.geom_type[g], &m.geom_size[3 * g], &m.geom_aabb[6 * g]);
+  for (int g = 0; g < ng; g++) {
+    local_aabb(m.geom_type[g], &m.geom_size[3 * g], &m.geom_aabb[6 * g]);
+    if (m.geom_type[g] == PH_GEOM_MESH) {  // model.rs:375-380: the hull's box, centre and half-extents
+      const int k = m.geom_dataid[g];
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (int v = m.mesh_vertadr[k]; v < m.mesh_vertadr[k] + m.mesh_vertnum[k]; v++)
+        for (int c = 0; c < 3; c++) {
+          const double x = m.mesh_vert[3 * v + c];
+          lo[c] = x < lo[c] ? x : lo[c];
+          hi[c] = x > hi[c] ? x : hi[c];
+        }
+      for (int c = 0; c < 3; c++) {
+        m.geom_aabb[6 * g + c] = (lo[c] + hi[c]) * 0.5;
+        m.geom_aabb[6 * g + 3 + c] = (hi[c] - lo[c]) * 0.5;
+      }
+    }
+  }
   // candidate pairs in lexicographic order (model.rs:398-407)
   m.pair_g1.clear();
   m.pair_g2.clear();
@@ -326,12 +419,8 @@ bool build_host_model(const ph_model_desc& s, HostModel& m, std::string* err) {
     for (int g2 = g1 + 1; g2 < ng; g2++) {
       if (!pair_static_ok(m, g1, g2)) continue;
       if (needs_general(m.geom_type[g1], m.geom_type[g2])) {
-        // cylinder-vs-non-plane pairs take the general convex narrow phase
-        // (dgeneral.hpp); mesh geoms have no hull on the device
-        if (m.geom_type[g1] == PH_GEOM_MESH || m.geom_type[g2] == PH_GEOM_MESH) {
-          *err = "mesh geoms need convex hulls, which this backend does not build";
-          return false;
-        }
+        // cylinder-vs-non-plane and mesh pairs take the general convex
+        // narrow phase (dgeneral.hpp)
         m.ngeneral++;
       }
       m.pair_g1.push_back(g1);

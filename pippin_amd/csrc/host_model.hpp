@@ -27,6 +27,13 @@ struct HostModel {
       cam_quat, cam_fovy, actuator_gear, actuator_gain, actuator_bias, actuator_ctrlrange, actuator_forcerange,
       moment_coef, eq_polycoef, eq_solref, eq_solimp, qpos0, qpos_reset;
 
+  // mesh hulls (raw), then what the narrow phase reads: a unit normal per face
+  // (cross(v1 - v0, v2 - v0) over its length) and the faces around each hull
+  // vertex (global vertex index), in increasing face order
+  std::vector<int32_t> mesh_vertadr, mesh_vertnum, mesh_faceadr, mesh_facenum, face_vertadr, face_vertnum, face_vert;
+  std::vector<double> mesh_vert, face_normal;
+  std::vector<int32_t> vert_faceadr, vert_facenum, vert_face;
+
   // derived (model.rs:279-316)
   std::vector<int32_t> body_lastdof, dof_parent, body_weldid, block_adr, dof_block;
   std::vector<uint64_t> body_chain, dof_anc;

@@ -340,6 +340,25 @@ PHD void passive_actuation(const MT& M, Env& E) {
   });
 }
 
+// The hull of geom g (a mesh geom), or an empty reference.
+template <class MT>
+PHD HullRef hull_ref(const MT& M, int g) {
+  HullRef h{};
+  const int k = M.geom_dataid[g];
+  if (k < 0) return h;
+  const int va = M.mesh_vertadr[k];
+  h.vert = M.mesh_vert + 3 * (int64_t)va;
+  h.vfadr = M.vert_faceadr + va;
+  h.vfnum = M.vert_facenum + va;
+  h.vface = M.vert_face;
+  h.fnormal = M.face_normal;
+  h.fvadr = M.face_vertadr;
+  h.fvnum = M.face_vertnum;
+  h.fvert = M.face_vert;
+  h.nvert = M.mesh_vertnum[k];
+  return h;
+}
+
 // ---------------- collision, forward.rs:214-260 ----------------
 template <class X, class MT>
 PHD void collide(const MT& M, const DState& S, Env& E) {
@@ -424,6 +443,10 @@ PHD void collide(const MT& M, const DState& S, Env& E) {
     Sink sink;
     sink.out = stage + (int64_t)p * MAXHIT * CW;
     sink.n = 0;
+    if (M.ngeneral > 0 && (a.typ == G_MESH || b.typ == G_MESH)) {
+      a.hull = hull_ref(M, g1);
+      b.hull = hull_ref(M, g2);
+    }
     if (M.ngeneral > 0 && needs_general(a.typ, b.typ)) collide_general(a, b, margin, sink);
     else collide_pair(a, b, margin, sink);
     cnt[p] = sink.n;

@@ -17,6 +17,7 @@
 
 #include "../../include/pippin_hip.h"
 #include "host_model.hpp"
+#include "meshes.hpp"
 
 namespace {
 
@@ -190,6 +191,9 @@ Mat madd(const Mat& a, const Mat& b) {
 }
 Mat mdiag(V d) { return {d.x, 0, 0, 0, d.y, 0, 0, 0, d.z}; }
 Mat mrotate(const Mat& m, const Mat& r) { return mmul(mmul(r, m), mT(r)); }
+V mulv(const Mat& r, V v) {
+  return {r[0] * v.x + r[1] * v.y + r[2] * v.z, r[3] * v.x + r[4] * v.y + r[5] * v.z, r[6] * v.x + r[7] * v.y + r[8] * v.z};
+}
 Q qnormalized(Q q) {
   double s = -0.0;
   s += q.w * q.w;
@@ -267,6 +271,8 @@ typedef std::map<std::string, std::map<std::string, std::string>> DefaultClass;
 
 struct Ctx {
   std::map<std::string, std::vector<float>> materials;
+  std::vector<std::string> mesh_names;  // index = geom_dataid
+  std::vector<phh::Hull> hulls;
   bool degrees = true;
   std::string eulerseq = "xyz";
   bool autolimits = true;
@@ -543,8 +549,15 @@ void parse_geom(Ctx& ctx, Builder& b, const XNode& node, int body, const std::st
   else if (ts == "capsule") gt = PH_GEOM_CAPSULE;
   else if (ts == "box") gt = PH_GEOM_BOX;
   else if (ts == "cylinder") gt = PH_GEOM_CYLINDER;
-  else if (ts == "mesh") throw LoadError{"mesh geoms need the convex-hull pipeline, not available in this loader"};
+  else if (ts == "mesh") gt = PH_GEOM_MESH;
   else return;  // unsupported geom types are skipped (mjcf.rs:645-648)
+  int dataid = -1;
+  if (gt == PH_GEOM_MESH) {
+    const char* mn = ctx.get(node, cls, "mesh");
+    if (!mn) throw LoadError{"mesh geom needs a 'mesh' attribute"};
+    dataid = find_name(ctx.mesh_names, mn);
+    if (dataid < 0) throw LoadError{std::string("unknown mesh '") + mn + "'"};
+  }
   std::vector<double> sz;
   ctx.floats(node, cls, "size", sz);
   auto get = [&](size_t i) { return i < sz.size() ? sz[i] : 0.0; };
@@ -576,7 +589,22 @@ void parse_geom(Ctx& ctx, Builder& b, const XNode& node, int body, const std::st
   }
   if (rgba.empty()) rgba = {0.5f, 0.5f, 0.5f, 1.0f};
   int32_t group = as_i32(ctx.flt(node, cls, "group", 0.0));
-  if (gt != PH_GEOM_PLANE) {
+  if (gt == PH_GEOM_MESH) {  // mjcf.rs:686-703: the hull's solid mass properties
+    double vol, com[3], icom[9];
+    phh::mass_properties(ctx.hulls[dataid], &vol, com, icom);
+    std::vector<double> mv;
+    double mass = ctx.floats(node, cls, "mass", mv) && !mv.empty() ? mv[0] : ctx.flt(node, cls, "density", 1000.0) * vol;
+    if (mass > 0.0 && vol > 0.0) {
+      Mat r = to_mat(quat);
+      V c = vadd(pos, mulv(r, V{com[0], com[1], com[2]}));
+      Mat ig = mrotate(mscale(Mat(icom, icom + 9), mass / vol), r);
+      Mat cc = mskew(c);
+      Mat shifted = madd(ig, mscale(mmul(cc, mT(cc)), mass));
+      b.gm_mass[body] += mass;
+      b.gm_mc[body] = vadd(b.gm_mc[body], vscale(c, mass));
+      b.gm_second[body] = madd(b.gm_second[body], shifted);
+    }
+  } else if (gt != PH_GEOM_PLANE) {
     double vol;
     V unit;
     geom_volume_inertia(gt, size, &vol, &unit);
@@ -601,7 +629,7 @@ void parse_geom(Ctx& ctx, Builder& b, const XNode& node, int body, const std::st
   m.geom_conaffinity.push_back(conaffinity);
   m.geom_rgba.insert(m.geom_rgba.end(), rgba.begin(), rgba.end());
   m.geom_group.push_back(group);
-  m.geom_dataid.push_back(-1);
+  m.geom_dataid.push_back(dataid);
 }
 
 void parse_inertial(Ctx& ctx, Builder& b, const XNode& node, int body, const std::string& cls) {
@@ -826,14 +854,60 @@ void parse_equality(Ctx& ctx, Builder& b, const XNode& node) {
   m.eq_solimp.insert(m.eq_solimp.end(), solimp, solimp + 5);
 }
 
+// mjcf.rs:740-772: a mesh asset's points (OBJ or STL file, or the 'vertex'
+// attribute), scaled per axis, and their convex hull.
+void parse_mesh(Ctx& ctx, const XNode& node, const std::string& meshdir) {
+  const char* file = node.get("file");
+  std::string name;
+  if (const char* nm = node.get("name")) name = nm;
+  else if (file) {
+    std::string f = file;
+    size_t slash = f.find_last_of('/');
+    f = slash == std::string::npos ? f : f.substr(slash + 1);
+    size_t dot = f.find_last_of('.');
+    name = dot == std::string::npos || dot == 0 ? f : f.substr(0, dot);
+  } else
+    throw LoadError{"mesh needs a name or file"};
+  V sc = ctx.vec3(node, "main", "scale", V{1, 1, 1});
+  const double scale[3] = {sc.x, sc.y, sc.z};
+  std::vector<double> pts;
+  if (file) {
+    std::string path = (file[0] == '/' ? std::string() : meshdir + "/") + file;
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw LoadError{"cannot read mesh '" + path + "'"};
+    std::stringstream ss;
+    ss << in.rdbuf();
+    std::string ext = path.size() > 4 ? path.substr(path.size() - 4) : "";
+    for (char& ch : ext) ch = (char)tolower((unsigned char)ch);
+    bool ok = ext == ".obj" ? phh::read_obj_vertices(ss.str(), scale, &pts)
+                            : ext == ".stl" ? phh::read_stl_vertices(ss.str(), scale, &pts) : false;
+    if (ext != ".obj" && ext != ".stl") throw LoadError{"unsupported mesh format (use .stl or .obj): " + path};
+    if (!ok) throw LoadError{"cannot read mesh '" + path + "'"};
+  } else {
+    std::vector<double> v;
+    if (!ctx.floats(node, "main", "vertex", v)) throw LoadError{"mesh '" + name + "' needs 'file' or 'vertex'"};
+    for (size_t i = 0; i + 2 < v.size(); i += 3)
+      for (int k = 0; k < 3; k++) pts.push_back(v[i + k] * scale[k]);
+  }
+  phh::Hull h;
+  std::string err;
+  if (!phh::convex_hull(pts, &h, &err)) throw LoadError{"mesh '" + name + "': " + err};
+  ctx.mesh_names.push_back(name);
+  ctx.hulls.push_back(h);
+}
+
 // mjcf.rs:211-395
-void load(const std::string& xml, phh::HostModel& m) {
+void load(const std::string& xml, const std::string& dir, phh::HostModel& m) {
   XmlParser parser(xml);
   std::unique_ptr<XNode> root = parser.parse();
   if (root->tag != "mujoco") throw LoadError{"root element must be <mujoco>"};
   Ctx ctx;
+  std::string meshdir = dir.empty() ? "." : dir;
   for (auto& c : root->kids) {
     if (c->tag != "compiler") continue;
+    const char* md = c->get("meshdir");
+    if (!md) md = c->get("assetdir");
+    if (md) meshdir = (md[0] == '/' ? std::string() : meshdir + "/") + md;
     if (const char* a = c->get("angle")) ctx.degrees = std::string(a) == "degree";
     if (const char* e = c->get("eulerseq")) {
       if (strlen(e) != 3) throw LoadError{"eulerseq must have 3 characters"};
@@ -905,7 +979,10 @@ void load(const std::string& xml, phh::HostModel& m) {
   for (auto& a : root->kids) {
     if (a->tag != "asset") continue;
     for (auto& c : a->kids) {
-      if (c->tag == "mesh") throw LoadError{"mesh assets need the convex-hull pipeline, not available in this loader"};
+      if (c->tag == "mesh") {
+        parse_mesh(ctx, *c, meshdir);
+        continue;
+      }
       if (c->tag != "material") continue;
       const char* k = c->get("class");
       std::string cls = k ? k : "main";
@@ -958,6 +1035,22 @@ void load(const std::string& xml, phh::HostModel& m) {
   d.nexclude = (int)m.exclude_pairs.size() / 2;
   d.nmoment = (int)m.moment_jnt.size();
   m.qpos_reset = m.qpos0;
+  for (const phh::Hull& h : ctx.hulls) {  // the descriptor's hull arrays
+    m.mesh_vertadr.push_back((int)m.mesh_vert.size() / 3);
+    m.mesh_vertnum.push_back((int)h.vert.size() / 3);
+    m.mesh_vert.insert(m.mesh_vert.end(), h.vert.begin(), h.vert.end());
+    m.mesh_faceadr.push_back((int)m.face_vertadr.size());
+    m.mesh_facenum.push_back((int)h.faces.size());
+    for (const auto& f : h.faces) {
+      m.face_vertadr.push_back((int)m.face_vert.size());
+      m.face_vertnum.push_back((int)f.size());
+      m.face_vert.insert(m.face_vert.end(), f.begin(), f.end());
+    }
+  }
+  d.nmesh = (int)ctx.hulls.size();
+  d.nmeshvert = (int)m.mesh_vert.size() / 3;
+  d.nmeshface = (int)m.face_vertadr.size();
+  d.nmeshfacevert = (int)m.face_vert.size();
   phh::retarget(m);
 }
 
@@ -970,14 +1063,13 @@ struct ph_model {
 };
 
 extern "C" int pippin_hip_model_load_xml(const char* xml, const char* dir, ph_model** out) {
-  (void)dir;
   if (!xml || !out) {
     ph_set_error("null argument");
     return PH_ERR_ARG;
   }
   std::unique_ptr<ph_model> m(new ph_model());
   try {
-    load(xml, m->hm);
+    load(xml, dir ? dir : ".", m->hm);
   } catch (const XmlError& e) {
     ph_set_error("xml parse error: " + e.msg);
     return PH_ERR_MODEL;

@@ -127,6 +127,7 @@ static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
 #define HIPCHK(x)                                                                         \
   do {                                                                                    \
     hipError_t e_ = (x);                                                                  \
@@ -206,8 +207,8 @@ struct ph_sim {
   std::vector<int> env_var;  // host mirror
   std::vector<int> var_refs;  // envs per variant
   // device buffers
-  DBuf<int> d_i[40];
-  DBuf<double> d_f[24];
+  DBuf<int> d_i[56];
+  DBuf<double> d_f[32];
   DBuf<uint64_t> d_u64[2];
   DBuf<uint32_t> d_u32[2];
   DBuf<int> d_env_var;
@@ -407,8 +408,13 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   HIPCHK(s->d_counter.zero());
 
   int ii = 0, fi = 0;
-  auto ui = [&](const std::vector<int32_t>& v, gp<const int>* dst) -> hipError_t { return up(s->d_i[ii++], v, dst); };
-  auto uf = [&](const std::vector<double>& v, gp<const Real>* dst) -> hipError_t { return up(s->d_f[fi++], v, dst); };
+  constexpr int NI = sizeof(s->d_i) / sizeof(s->d_i[0]), NF = sizeof(s->d_f) / sizeof(s->d_f[0]);
+  auto ui = [&](const std::vector<int32_t>& v, gp<const int>* dst) -> hipError_t {
+    return ii < NI ? up(s->d_i[ii++], v, dst) : hipErrorInvalidValue;
+  };
+  auto uf = [&](const std::vector<double>& v, gp<const Real>* dst) -> hipError_t {
+    return fi < NF ? up(s->d_f[fi++], v, dst) : hipErrorInvalidValue;
+  };
   HIPCHK(ui(hm.body_parent, &M.body_parent));
   HIPCHK(ui(hm.body_jntadr, &M.body_jntadr));
   HIPCHK(ui(hm.body_jntnum, &M.body_jntnum));
@@ -463,6 +469,17 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
   HIPCHK(uf(hm.eq_solref, &M.eq_solref));
   HIPCHK(uf(hm.eq_solimp, &M.eq_solimp));
   HIPCHK(uf(hm.qpos0, &M.qpos0));
+  HIPCHK(ui(hm.geom_dataid, &M.geom_dataid));
+  HIPCHK(ui(hm.mesh_vertadr, &M.mesh_vertadr));
+  HIPCHK(ui(hm.mesh_vertnum, &M.mesh_vertnum));
+  HIPCHK(ui(hm.face_vertadr, &M.face_vertadr));
+  HIPCHK(ui(hm.face_vertnum, &M.face_vertnum));
+  HIPCHK(ui(hm.face_vert, &M.face_vert));
+  HIPCHK(ui(hm.vert_faceadr, &M.vert_faceadr));
+  HIPCHK(ui(hm.vert_facenum, &M.vert_facenum));
+  HIPCHK(ui(hm.vert_face, &M.vert_face));
+  HIPCHK(uf(hm.mesh_vert, &M.mesh_vert));
+  HIPCHK(uf(hm.face_normal, &M.face_normal));
   HIPCHK(up(s->d_u64[0], hm.body_chain, &M.body_chain));
   HIPCHK(up(s->d_u64[1], hm.dof_anc, &M.dof_anc));
 

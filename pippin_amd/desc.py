@@ -52,6 +52,22 @@ ARRAY_FIELDS = [
 ]
 
 SCALAR_INTS = ["nq", "nv", "nu", "nbody", "njnt", "ngeom", "ncam", "neq", "nexclude", "nmoment"]
+# mesh hulls: counts, then arrays (name, ctype, dtype, width, count field)
+MESH_INTS = ["nmesh", "nmeshvert", "nmeshface", "nmeshfacevert"]
+MESH_FIELDS = [
+    ("mesh_vertadr", _i32p, np.int32, 1, "nmesh"), ("mesh_vertnum", _i32p, np.int32, 1, "nmesh"),
+    ("mesh_faceadr", _i32p, np.int32, 1, "nmesh"), ("mesh_facenum", _i32p, np.int32, 1, "nmesh"),
+    ("mesh_vert", _f64p, np.float64, 3, "nmeshvert"), ("face_vertadr", _i32p, np.int32, 1, "nmeshface"),
+    ("face_vertnum", _i32p, np.int32, 1, "nmeshface"), ("face_vert", _i32p, np.int32, 1, "nmeshfacevert"),
+]
+
+
+def no_meshes() -> dict:
+    """The mesh fields of a model without mesh assets."""
+    out = {n: 0 for n in MESH_INTS}
+    for name, _c, dtype, width, _n in MESH_FIELDS:
+        out[name] = np.zeros((0, width) if width > 1 else 0, dtype=dtype)
+    return out
 
 
 class ph_model_desc(C.Structure):
@@ -64,6 +80,8 @@ class ph_model_desc(C.Structure):
            ("contact_hertz", C.c_double), ("contact_damping_ratio", C.c_double),
            ("contact_slop", C.c_double)]
         + [(n, t) for (n, t, _, _) in ARRAY_FIELDS]
+        + [(n, C.c_int32) for n in MESH_INTS]
+        + [(n, t) for (n, t, _, _, _) in MESH_FIELDS]
     )
 
 
@@ -127,6 +145,15 @@ class Desc:
                 a = np.zeros(1, dtype=dtype)  # keep a valid pointer for empty arrays
             self.arrays[name] = a
             setattr(d, name, a.ctypes.data_as(ctype))
+        mesh = arrays if "nmesh" in arrays else no_meshes()
+        for n in MESH_INTS:
+            setattr(d, n, int(mesh[n]))
+        for name, ctype, dtype, width, _cnt in MESH_FIELDS:
+            a = np.ascontiguousarray(np.asarray(mesh[name], dtype=dtype).reshape(-1))
+            if a.size == 0:
+                a = np.zeros(1, dtype=dtype)
+            self.arrays[name] = a
+            setattr(d, name, a.ctypes.data_as(ctype))
         self.struct = d
         self.scalars = {k: arrays[k] for k in arrays if k not in self.arrays}
 
@@ -161,6 +188,12 @@ def arrays_from_desc(d: ph_model_desc) -> dict:
         elif name in ("moment_jnt", "moment_coef"):
             n = d.nmoment
         total = n * width
+        p = getattr(d, name)
+        out[name] = np.ctypeslib.as_array(p, shape=(total,)).copy() if total > 0 else np.zeros(0, dtype)
+    for n in MESH_INTS:
+        out[n] = int(getattr(d, n))
+    for name, _ctype, dtype, width, cnt in MESH_FIELDS:
+        total = out[cnt] * width
         p = getattr(d, name)
         out[name] = np.ctypeslib.as_array(p, shape=(total,)).copy() if total > 0 else np.zeros(0, dtype)
     return out

@@ -157,6 +157,17 @@ emu_sim* emu_create(const ph_model_desc* desc, int64_t n) {
   M.eq_solref = hm.eq_solref.data();
   M.eq_solimp = hm.eq_solimp.data();
   M.qpos0 = hm.qpos0.data();
+  M.geom_dataid = hm.geom_dataid.data();
+  M.mesh_vertadr = hm.mesh_vertadr.data();
+  M.mesh_vertnum = hm.mesh_vertnum.data();
+  M.mesh_vert = hm.mesh_vert.data();
+  M.face_normal = hm.face_normal.data();
+  M.face_vertadr = hm.face_vertadr.data();
+  M.face_vertnum = hm.face_vertnum.data();
+  M.face_vert = hm.face_vert.data();
+  M.vert_faceadr = hm.vert_faceadr.data();
+  M.vert_facenum = hm.vert_facenum.data();
+  M.vert_face = hm.vert_face.data();
   s->env_var.assign(n, 0);
   M.env_var = s->env_var.data();
   s->v_size = hm.geom_size;

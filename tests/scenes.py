@@ -119,3 +119,45 @@ GRIPPER = """<mujoco model="gripper">
 
 
 from pippin_amd.workloads import random_pile_xml  # noqa: E402,F401  (shared with bench.py)
+
+
+# mesh geoms (convex hulls): the reference's mesh cube (crates/pippin/tests/mesh.rs)
+# and the 12-vertex wedge of the tumble workload, against a plane, every
+# primitive and each other; full documents (they need an <asset> section)
+MESH_CUBE = ("-0.05 -0.05 -0.05  0.05 -0.05 -0.05  -0.05 0.05 -0.05  0.05 0.05 -0.05 "
+             "-0.05 -0.05 0.05  0.05 -0.05 0.05  -0.05 0.05 0.05  0.05 0.05 0.05")
+
+
+def _wedge_vertex():
+    from pippin_amd.workloads import wedge_vertices
+    return " ".join("%.6f %.6f %.6f" % v for v in wedge_vertices())
+
+
+def mesh_xml(name: str) -> str:
+    assets = f'<asset><mesh name="cube" vertex="{MESH_CUBE}"/><mesh name="wedge" vertex="{_wedge_vertex()}"/></asset>'
+    return f'<mujoco><option timestep="0.004"/>{assets}<worldbody>{MESH_SCENES[name]}</worldbody></mujoco>'
+
+
+MESH_SCENES = {
+    "cube_mesh_and_box": '<geom type="plane" size="5 5 .1"/>'
+                         '<body pos="0 0 0.2" euler="4 7 0"><freejoint/><geom type="mesh" mesh="cube"/></body>'
+                         '<body pos="0.5 0 0.2" euler="4 7 0"><freejoint/><geom type="box" size=".05 .05 .05"/></body>',
+    "mesh_cubes_stack": '<geom type="plane" size="5 5 .1"/>'
+                        '<body pos="0 0 .06"><freejoint/><geom type="mesh" mesh="cube"/></body>'
+                        '<body pos=".01 0 .17" euler="0 0 20"><freejoint/><geom type="mesh" mesh="cube"/></body>'
+                        '<body pos="-.01 .01 .28" euler="0 0 40"><freejoint/><geom type="mesh" mesh="cube"/></body>',
+    "wedge_on_plane": '<geom type="plane" size="5 5 .1"/>'
+                      '<body pos="0 0 0.15" euler="20 35 10"><freejoint/><geom type="mesh" mesh="wedge"/></body>',
+    "primitives_on_wedge": '<geom type="plane" size="5 5 .1"/>'
+                           '<body pos="0 0 0.03" euler="0 0 15"><freejoint/><geom type="mesh" mesh="wedge" density="3000"/></body>'
+                           '<body pos="-.01 .005 .1"><freejoint/><geom type="sphere" size=".015"/></body>'
+                           '<body pos=".02 -.01 .16" euler="80 10 0"><freejoint/><geom type="capsule" size=".01 .02"/></body>'
+                           '<body pos="0 0 .22" euler="5 85 0"><freejoint/><geom type="cylinder" size=".012 .02"/></body>'
+                           '<body pos=".005 .01 .28" euler="30 20 10"><freejoint/><geom type="box" size=".015 .012 .01"/></body>',
+    "wedge_on_box_and_wedge": '<geom type="box" size="1 1 .1" pos="0 0 -.1"/>'
+                              '<body pos="0 0 0.03"><freejoint/><geom type="mesh" mesh="wedge"/></body>'
+                              '<body pos=".01 .005 0.12" euler="180 10 30"><freejoint/><geom type="mesh" mesh="wedge"/></body>',
+    # the plane listed after the mesh body: the pair is (mesh, plane)
+    "mesh_before_plane": '<body pos="0 0 0.2" euler="10 -20 5"><freejoint/><geom type="mesh" mesh="cube"/></body>'
+                         '<geom type="plane" size="5 5 .1"/>',
+}

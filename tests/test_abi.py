@@ -61,9 +61,12 @@ def _generated_models():
     out += [(f"pile {s}", random_pile_xml(s)) for s in (11, 29)]
     out += [(f"general pile {s}", random_pile_xml(s, general=True)) for s in (13, 18)]
     out += [("arm gripper 5", arm_gripper_xml(5)), ("arm gripper 0", arm_gripper_xml(0))]
-    from tests.scenes import GRIPPER
+    from pippin_amd.workloads import tabletop_tumble
+    from tests.scenes import GRIPPER, MESH_SCENES, mesh_xml
 
     out.append(("gripper", GRIPPER))
+    out += [(f"mesh {k}", mesh_xml(k)) for k in sorted(MESH_SCENES)]
+    out.append(("tabletop tumble", tabletop_tumble(2, 0).xml))
     return out
 
 
@@ -96,3 +99,45 @@ def test_create_without_gpu_fails_loudly(assets):
     arr = physics.load_model(assets / "ant.xml")
     with pytest.raises(physics.HipError):
         physics.HipPhysics(arr, 4)
+
+
+def test_library_loader_reads_obj_and_stl_meshes(tmp_path):
+    """Mesh files: an OBJ (the reference's wedge test, tests/mesh.rs:85-116) and
+    the same points as binary and ASCII STL give the same hull through both
+    loaders."""
+    import struct
+
+    pts = [(0, 0, 0), (0.1, 0, 0), (0, 0.1, 0), (0, 0, 0.1), (0.1, 0.1, 0)]
+    tris = [(0, 2, 1), (1, 2, 4), (0, 1, 3), (0, 3, 2), (1, 4, 3), (2, 3, 4)]
+    (tmp_path / "wedge.obj").write_text("".join("v %g %g %g\n" % p for p in pts)
+                                        + "".join("f %d %d %d\n" % tuple(i + 1 for i in t) for t in tris))
+    binary = b"\0" * 80 + struct.pack("<I", len(tris))
+    for t in tris:
+        binary += struct.pack("<3f", 0, 0, 0) + b"".join(struct.pack("<3f", *pts[i]) for i in t) + b"\0\0"
+    (tmp_path / "wedge_b.stl").write_bytes(binary)
+    (tmp_path / "wedge_a.stl").write_text(
+        "solid w\n" + "".join("facet normal 0 0 0\nouter loop\n" + "".join("vertex %g %g %g\n" % pts[i] for i in t)
+                               + "endloop\nendfacet\n" for t in tris) + "endsolid w\n")
+    hulls = []
+    for f in ("wedge.obj", "wedge_b.stl", "wedge_a.stl"):
+        xml = (f'<mujoco><asset><mesh name="w" file="{f}" scale="2 2 2"/></asset><worldbody>'
+               '<body><freejoint/><geom type="mesh" mesh="w"/></body></worldbody></mujoco>')
+        (tmp_path / "m.xml").write_text(xml)
+        a = physics.load_model(tmp_path / "m.xml")
+        b = mjcf.load_file(tmp_path / "m.xml")
+        for k in b:
+            assert np.array_equal(np.asarray(a[k]).reshape(-1), np.asarray(b[k]).reshape(-1).astype(np.asarray(a[k]).dtype)), (f, k)
+        hulls.append((np.asarray(a["mesh_vert"]).copy(), np.asarray(a["face_vert"]).copy(), np.asarray(a["body_mass"])))
+        assert abs(np.asarray(a["mesh_vert"]).reshape(-1, 3)[:, 0].max() - 0.2) < 1e-6
+    for h in hulls[1:]:
+        assert all(np.array_equal(x, y) for x, y in zip(h, hulls[0]))
+
+
+def test_degenerate_mesh_is_refused():
+    flat = "0 0 0  1 0 0  0 1 0  1 1 0"
+    xml = (f'<mujoco><asset><mesh name="f" vertex="{flat}"/></asset><worldbody>'
+           '<body><freejoint/><geom type="mesh" mesh="f"/></body></worldbody></mujoco>')
+    with pytest.raises(physics.HipError, match="degenerate"):
+        physics.load_model(xml, is_xml=True)
+    with pytest.raises(mjcf.MjcfError, match="degenerate"):
+        mjcf.load_string(xml)

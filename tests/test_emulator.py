@@ -13,7 +13,8 @@ import pytest
 from oracle import mjcf
 from oracle.pyoracle import OracleSim
 from tests.emu.pyemu import EmuSim
-from tests.scenes import GENERAL_SCENES, MULTIBODY_SCENES, PRIMITIVE_SCENES, general_xml, primitive_xml
+from tests.scenes import (GENERAL_SCENES, MESH_SCENES, MULTIBODY_SCENES, PRIMITIVE_SCENES, general_xml, mesh_xml,
+                          primitive_xml)
 
 
 def _perturbed(q, rng, scale, free=True):
@@ -395,3 +396,71 @@ def test_arm_gripper_exact(solver):
         assert np.array_equal(o.ncon(), e.ncon()), t
         assert np.array_equal(o.get("qpos"), e.get("qpos")), t
     assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("name", sorted(MESH_SCENES))
+@pytest.mark.parametrize("solver", [0, 1])
+def test_mesh_scenes_exact(name, solver):
+    """Convex hulls (plane against a hull, hulls against every primitive and
+    each other) in the kernel body against the oracle, bit for bit."""
+    arr = mjcf.load_string(mesh_xml(name))
+    arr["solver_kind"] = solver
+    n = 3
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(23)
+    q = o.get("qpos")
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+    for t in range(150):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    assert o.ncon().sum() > 0
+
+
+def _bake(arr, plan, env):
+    """The tumble model with env `env`'s per-env parameters written into its
+    arrays (the emulator keeps one parameter set per model)."""
+    out = {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for k, v in arr.items()}
+    for name, gid, values in plan.params:
+        v = values[env]
+        if name in ("geom_contype", "geom_conaffinity"):
+            out[name][gid] = int(v[0])
+        elif name == "geom_size":
+            out[name][gid] = v
+        elif name in ("geom_friction", "body_mass"):
+            out[name][gid] = v[0]
+        elif name == "body_inertia":
+            out[name][gid] = np.diag(v).reshape(out[name][gid].shape)
+        elif name == "body_ipos":
+            out[name][gid] = v
+    out["qpos_reset"] = plan.qpos0[env].copy()
+    return out
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_tabletop_tumble_exact(solver):
+    """The tumble workload (2 to 6 objects per env chosen by contype masks, hull
+    wedges among the primitives) in the kernel body against the oracle, one
+    env per model with its parameters baked in."""
+    from pippin_amd.workloads import tabletop_tumble
+
+    n = 4
+    plan = tabletop_tumble(n, seed=solver + 5)
+    base = mjcf.load_string(plan.xml)
+    base["solver_kind"] = solver
+    total = 0
+    for env in range(n):
+        arr = _bake(base, plan, env)
+        o, e = OracleSim(arr, 1), EmuSim(arr, 1)
+        assert np.array_equal(o.get("qpos"), e.get("qpos"))
+        for t in range(120):
+            for s in (o, e):
+                s.step(0, 1, 1)
+            assert np.array_equal(o.ncon(), e.ncon()), (env, t)
+            assert np.array_equal(o.get("qpos"), e.get("qpos")), (env, t)
+            total += int(o.ncon().sum())
+    assert total > 0

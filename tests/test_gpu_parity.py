@@ -15,7 +15,8 @@ import pytest
 from oracle import mjcf
 from oracle.pyoracle import OracleSim, cr_sincos
 from pippin_amd.physics import HipPhysics, load_model, selftest_math
-from tests.scenes import GENERAL_SCENES, MULTIBODY_SCENES, PRIMITIVE_SCENES, general_xml, primitive_xml
+from tests.scenes import (GENERAL_SCENES, MESH_SCENES, MULTIBODY_SCENES, PRIMITIVE_SCENES, general_xml, mesh_xml,
+                          primitive_xml)
 
 pytestmark = pytest.mark.gpu
 
@@ -416,6 +417,56 @@ def test_arm_gripper_with_per_env_cubes_match(solver):
     for t in range(200):
         for s in (o, g):
             s.set("ctrl", 0, n, tg[t])
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("name", sorted(MESH_SCENES))
+@pytest.mark.parametrize("solver", ["newton", "pgs"])
+def test_mesh_scenes_match(name, solver):
+    """Convex hulls on the device: plane against hull (either order), hulls
+    against every primitive and each other, models built by the library's own
+    loader, against the oracle bit for bit."""
+    arr = load_model(mesh_xml(name), is_xml=True)  # the library's own loader
+    arr["solver_kind"] = 0 if solver == "newton" else 1
+    n = 12
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(29)
+    q = o.get("qpos")
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    say("mesh scene", name, solver, n)
+    for t in range(200):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+    assert np.array_equal(o.solver_iterations(), g.solver_iterations())
+    assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("solver", ["newton", "pgs"])
+def test_tabletop_tumble_match(solver):
+    """The tumble workload: per-env object sets (2 to 6 of box, sphere,
+    cylinder, capsule and hull wedge, by contype masks), per-env size,
+    density and friction, drops from per-env poses; 64 envs, 300 steps."""
+    from pippin_amd.workloads import tabletop_tumble
+
+    n = 64
+    plan = tabletop_tumble(n, seed=3)
+    arr = load_model(plan.xml, is_xml=True)
+    arr["solver_kind"] = 0 if solver == "newton" else 1
+    o, g = OracleSim(arr, n), HipPhysics(arr, n, max_contacts=96)
+    for s in (o, g):
+        plan.apply(s)
+    assert np.array_equal(o.get("qpos"), g.get("qpos"))
+    say("tabletop tumble", solver, n)
+    for t in range(300):
+        for s in (o, g):
             s.step(0, n, 1)
         assert np.array_equal(o.ncon(), g.ncon()), t
         assert np.array_equal(o.get("qpos"), g.get("qpos")), t
