@@ -271,6 +271,84 @@ def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = Fal
     return out
 
 
+def tumble_study(n: int, steps: int, seed: int, cpu_envs: int):
+    """The tabletop tumble configuration (pippin_amd/workloads.py
+    tabletop_tumble): per env 2 to 6 objects from boxes, spheres, cylinders,
+    capsules and 12-vertex hull wedges, per-env sizes, densities U[500, 1500]
+    and frictions U[0.5, 1.0] set through per-env parameters, dropped from
+    non-overlapping poses 0.05 to 0.2 m above a plane; `steps` physics steps
+    from the drop, both solvers. The launch shape is autotuned on a separate
+    sim of the same envs first (the choice is cached and taken by the timed
+    sim). The oracle steps the first `cpu_envs` envs over the same steps on the
+    host cores; its end state must equal the GPU's bit for bit."""
+    import tempfile
+
+    import torch
+    from oracle.pyoracle import OracleSim
+    from pippin_amd.physics import HipError, HipPhysics, load_model
+    from pippin_amd.workloads import tabletop_tumble
+
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    plan = tabletop_tumble(n, seed)
+    gen_s = time.perf_counter() - t0
+    arrays = load_model(plan.xml, is_xml=True)
+    cache = os.path.join(tempfile.mkdtemp(), "shapes.txt")
+    out = []
+    for solver in ("newton", "pgs"):
+        failures = []
+
+        def step(sim, nstep, where):
+            try:
+                sim.step(0, n, nstep)
+            except HipError as ex:
+                failures.append(f"{where}: {ex}")
+
+        tuner = HipPhysics(arrays, n, solver=solver, max_contacts=96)
+        plan.apply(tuner)
+        tuner.autotune(rounds=1, cache_path=cache)
+        tuned = 0
+        while tuner.autotune_pending() and tuned < 64:
+            step(tuner, 4, f"autotune launch {tuned}")
+            tuned += 1
+        tuner.close()
+        sim = HipPhysics(arrays, n, solver=solver, max_contacts=96)
+        a0 = time.perf_counter()
+        plan.apply(sim)
+        torch.cuda.synchronize()
+        apply_s = time.perf_counter() - a0
+        sim.autotune(rounds=1, cache_path=cache)  # takes the cached shape
+        failures.clear()
+        sim.kernel_timing(reset=True)
+        t0 = time.perf_counter()
+        for li in range(steps // 4):
+            step(sim, 4, f"timed launch {li}")
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        kms, launches = sim.kernel_timing(reset=True)
+        m = min(cpu_envs, n)
+        ref = OracleSim(arrays, m)
+        plan.apply(ref, 0, m)
+        c0 = time.perf_counter()
+        ref.step(0, m, steps, threads=threads)
+        cdt = time.perf_counter() - c0
+        same_q = bool(np.array_equal(ref.get("qpos"), sim.get("qpos", 0, m)))
+        same_n = bool(np.array_equal(ref.ncon(), sim.ncon(0, m)))
+        out.append({"workload": f"tabletop tumble (seed {seed}), {solver}", "envs": n, "steps": steps,
+                    "gpu_env_steps_per_s": n * steps / dt, "kernel_ms_per_launch": kms / launches,
+                    "cpu_env_steps_per_s": m * steps / cdt, "cpu_envs": m, "cpu_threads": threads,
+                    "equal_to_oracle": {"envs": m, "qpos": same_q, "ncon": same_n},
+                    "objects_per_env_mean": float(plan.counts.mean()),
+                    "wedges": int((plan.kinds == 4).sum()),
+                    "contacts_mean": float(sim.ncon().mean()),
+                    "newton_iters_mean": float(sim.solver_iterations().mean()),
+                    "launch_shape": sim.launch_shape(), "param_apply_s": apply_s, "plan_s": gen_s,
+                    "env_failures": {"first": failures[0] if failures else None,
+                                     "launches_reporting_one": len(failures)}})
+        sim.close()
+    return out
+
+
 def arm_study(n: int, steps: int, seed: int, cpu_envs: int):
     """The arm configuration (pippin_amd/workloads.py arm_gripper_xml: 7-DoF
     arm, tendon-coupled two-finger gripper, cube on a table, implicitfast),
@@ -367,11 +445,18 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch every step eagerly instead of graph replay")
     ap.add_argument("--arm", action="store_true",
                     help="only run the arm study (4096 envs, rollout with per-step targets, GPU and host oracle)")
+    ap.add_argument("--tumble", action="store_true",
+                    help="only run the tabletop tumble study (8192 envs, 500 steps, both solvers, per-env object "
+                         "sets with hull wedges, GPU and host oracle) and print one JSON line per solver")
     ap.add_argument("--piles-general", action="store_true",
                     help="as --piles, on piles whose cylinders meet every body (general convex pairs)")
     args = ap.parse_args()
     if args.arm:
         print(json.dumps(arm_study(4096, 200, 0, 512)), flush=True)
+        return
+    if args.tumble:
+        for line in tumble_study(args.envs if args.envs != 16384 else 8192, 500, args.seed, 1024):
+            print(json.dumps(line), flush=True)
         return
     if args.piles or args.piles_general:
         seeds = (13, 18) if args.piles_general else (11, 29)
