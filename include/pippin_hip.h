@@ -207,6 +207,15 @@ int64_t pippin_hip_num_envs(const ph_sim* s);
  * kinematics rerun after step amounts to, pippin_cpu.rs:107-110). */
 int pippin_hip_step(ph_sim* s, int64_t begin, int64_t end, int64_t nstep);
 int pippin_hip_reset(ph_sim* s, const int64_t* envs, int64_t n);
+/* pippin_hip_reset for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops that reset finished envs every step,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. It calls
+ * no hipMalloc or hipFree: the env ids are staged in a fixed device array of
+ * 65536 ids (512 KiB per GPU, held until the process ends); a call with more
+ * ids returns PH_ERR_ARG and changes nothing. Every sim on one GPU shares that
+ * array, so calls that overlap in time on one GPU (other threads, other
+ * streams) race; calls from one thread are ordered. */
+int pippin_hip_reset_noalloc(ph_sim* s, const int64_t* envs, int64_t n);
 /* `out` / `src` hold (end - begin) x width doubles, width = dims.{nq,nv,nu} for
  * qpos/qvel/ctrl, nv for qacc and qfrc_applied, 1 for time. */
 int pippin_hip_get(const ph_sim* s, int32_t field, int64_t begin, int64_t end, double* out, int64_t len);

@@ -11,7 +11,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,6 +73,13 @@ __global__ void __launch_bounds__(64) k_reset(const DModel* __restrict__ Mp, con
   }
 }
 
+// Staging of env ids for pippin_hip_reset_noalloc: a file-scope device array,
+// so that entry point allocates nothing. Each GPU holds its own copy until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (scratch_on_device).
+static constexpr int64_t RESET_SCRATCH_MAX = 65536;  // env ids per call (512 KiB)
+static __device__ __align__(256) int64_t scratch[RESET_SCRATCH_MAX];
+
 // Forward dynamics up to RNE; copies M(q) and c(q, qdot) out.
 __global__ void __launch_bounds__(64) k_mass_bias(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
                                                   int64_t begin, Real* qm, Real* bias) {
@@ -128,6 +137,24 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller has made it current).
+static hipError_t scratch_on_device(int device, int64_t** out) {
+  static std::mutex mu;
+  static std::map<int, int64_t*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  addr[device] = static_cast<int64_t*>(p);
+  *out = static_cast<int64_t*>(p);
+  return hipSuccess;
+}
 #define HIPCHK(x)                                                                         \
   do {                                                                                    \
     hipError_t e_ = (x);                                                                  \
@@ -199,6 +226,7 @@ struct ph_sim {
   // set(Qpos), pippin_cpu.rs:107-110, 183-185); they are recomputed for all
   // envs on the first poses() call after qpos changed.
   mutable bool poses_stale = true;
+  int64_t* reset_scratch = nullptr;  // this device's `scratch` (pippin_hip_reset_noalloc)
   bool qpos_shared = false;  // a device qpos pointer was handed out (pippin_hip_state_device)
   hipStream_t stream = nullptr;
   DModel M;
@@ -393,6 +421,7 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
       HIPCHK(hipFuncSetAttribute(ws_kernel(k, 0), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     }
   }
+  HIPCHK(scratch_on_device(s->device, &s->reset_scratch));
   s->cus = prop.multiProcessorCount;
   s->arch = prop.gcnArchName;
   for (int ki = 0; ki < 3; ki++)
@@ -951,6 +980,27 @@ extern "C" int pippin_hip_reset(ph_sim* s, const int64_t* envs, int64_t n) {
   std::vector<int64_t> ids(envs, envs + n);
   HIPCHK(s->d_ids.upload(ids));
   hipLaunchKernelGGL(k_reset, dim3((unsigned)n), dim3(64), 0, s->stream, s->d_model.p, s->d_state.p, s->d_ids.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->poses_stale = true;
+  return PH_OK;
+}
+
+// pippin_hip_reset with the env ids staged in the file-scope `scratch` instead
+// of a buffer allocated per call.
+extern "C" int pippin_hip_reset_noalloc(ph_sim* s, const int64_t* envs, int64_t n) {
+  if (!s || (n > 0 && !envs)) return fail(PH_ERR_ARG, "null argument");
+  if (n <= 0) return PH_OK;
+  if (n > RESET_SCRATCH_MAX)
+    return fail(PH_ERR_ARG, "reset_noalloc takes at most " + std::to_string(RESET_SCRATCH_MAX) + " envs per call, got " +
+                                std::to_string(n));
+  for (int64_t i = 0; i < n; i++)
+    if (envs[i] < 0 || envs[i] >= s->n) return fail(PH_ERR_ARG, "env id out of range");
+  HIPCHK(hipSetDevice(s->device));
+  int64_t* ids = s->reset_scratch;
+  HIPCHK(hipMemcpyAsync(ids, envs, n * sizeof(int64_t), hipMemcpyHostToDevice, s->stream));
+  hipLaunchKernelGGL(k_reset, dim3((unsigned)n), dim3(64), 0, s->stream, s->d_model.p, s->d_state.p,
+                     (const int64_t*)ids);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   s->poses_stale = true;

@@ -65,6 +65,7 @@ def lib():
             "pippin_hip_num_envs": ([vp], i64),
             "pippin_hip_step": ([vp, i64, i64, i64], C.c_int),
             "pippin_hip_reset": ([vp, vp, i64], C.c_int),
+            "pippin_hip_reset_noalloc": ([vp, vp, i64], C.c_int),
             "pippin_hip_get": ([vp, i32, i64, i64, vp, i64], C.c_int),
             "pippin_hip_set": ([vp, i32, i64, i64, vp, i64], C.c_int),
             "pippin_hip_poses": ([vp, i64, i64, vp, i64, vp, i64], C.c_int),
