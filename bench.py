@@ -44,10 +44,11 @@ FP64_PEAK_TFLOPS = 78.6
 # reference FP64 ops per env-step on this workload, used only with --no-cpu
 OPS_PER_ENV_STEP_DEFAULT = 25000.0
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_step launch of the default workload (16384 envs x 4 substeps),
-# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a known-size copy
-# (profiles/k_step_ws_ant16k.md); other workloads report null.
-TRAFFIC_BYTES_PER_LAUNCH = 44.1e6
+# HBM bytes per k_step launch of the default workload (16384 envs x 4 substeps,
+# the steady-contact window), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# scaled by a known-size copy (profiles/k_step_ws_ant16k.md); other workloads
+# report null.
+TRAFFIC_BYTES_PER_LAUNCH = 51.7e6
 TRAFFIC_SOURCE = "profiles/k_step_ws_ant16k.md"
 PREROLL_ACTIONS = 250  # untimed: the timed window starts at action 250 of 1000 (steady contact)
 ERR_BOUND = 1e-9
@@ -252,22 +253,86 @@ def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = Fal
         dt = time.perf_counter() - t0
         kms, launches = sim.kernel_timing(reset=True)
         m = min(cpu_envs, n)
-        ref = OracleSim(arrays, m)
+        # the host runs the same solver (the model file names Newton)
+        ref = OracleSim({**arrays, "solver_kind": 0 if solver == "newton" else 1}, m)
         ref.set("qpos", 0, m, q[:m])
         ref.step(0, m, 40 + 4 * tuned, threads=threads)
         c0 = time.perf_counter()
         ref.step(0, m, steps, threads=threads)
         cdt = time.perf_counter() - c0
+        same = {"envs": m, "qpos": bool(np.array_equal(ref.get("qpos"), sim.get("qpos", 0, m))),
+                "ncon": bool(np.array_equal(ref.ncon(), sim.ncon(0, m)))}
         kind = ", cylinders meet every body" if general else ""
         out.append({"workload": f"pile seed {seed} ({nb} bodies{kind}), {solver}", "envs": n, "steps": steps,
                     "gpu_env_steps_per_s": n * steps / dt, "kernel_ms_per_launch": kms / launches,
                     "cpu_env_steps_per_s": m * steps / cdt, "cpu_envs": m, "cpu_threads": threads,
+                    "equal_to_oracle": same,
                     "newton_iters_mean": float(sim.solver_iterations().mean()),
                     "launch_shape": sim.launch_shape(),
                     "env_failures": {"first": failures[0] if failures else None,
                                      "launches_reporting_one": len(failures)},
                     "contacts_mean": float(sim.ncon().mean())})
         sim.close()
+    return out
+
+
+def mfma_study(n: int, steps: int, seed: int, cpu_envs: int):
+    """Newton's dense Hessian J'DJ on the FP64 matrix cores (ph_options.hessian_mfma)
+    against the reference's row-by-row sums, on a 6-body pile (nv 36; islands over
+    16 dofs take the dense solver), one env per wavefront so that the matrix cores
+    serve each env: step kernel time with and without, and how far the trajectory
+    of the first `cpu_envs` envs moves from the oracle's (which the plain path
+    matches bit for bit), every 20 steps."""
+    import torch
+    from oracle.pyoracle import OracleSim
+    from pippin_amd.physics import HipPhysics, load_model
+    from pippin_amd.workloads import random_pile_xml
+
+    arrays = load_model(random_pile_xml(seed), is_xml=True)
+    rng = np.random.default_rng(seed)
+    probe = HipPhysics(arrays, 1)
+    q1 = probe.get("qpos")[0]
+    probe.close()
+    q = np.tile(q1, (n, 1))
+    nb = int(arrays["nq"]) // 7
+    for b in range(nb):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    out = {"workload": f"pile seed {seed} ({nb} bodies, nv {int(arrays['nv'])}), Newton, {n} envs, "
+                       f"launch shape (1, generic, ws_bucket)", "steps": steps}
+    for flag in (False, True):
+        sim = HipPhysics(arrays, n, hessian_mfma=flag)
+        sim.set_launch_shape(1, 0, "ws_bucket")
+        sim.set("qpos", 0, n, q)
+        sim.step(0, n, 40)
+        torch.cuda.synchronize()
+        sim.kernel_timing(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(steps // 4):
+            sim.step(0, n, 4)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        kms, launches = sim.kernel_timing(reset=True)
+        key = "mfma" if flag else "plain"
+        out[f"{key}_env_steps_per_s"] = n * steps / dt
+        out[f"{key}_kernel_ms_per_launch"] = kms / launches
+        out[f"{key}_newton_iters_mean"] = float(sim.solver_iterations().mean())
+        sim.close()
+    m = min(cpu_envs, n)
+    ref = OracleSim(arrays, m)
+    dev = {f: HipPhysics(arrays, m, hessian_mfma=f) for f in (False, True)}
+    for f, d in dev.items():
+        d.set_launch_shape(1, 0, "ws_bucket")
+        d.set("qpos", 0, m, q[:m])
+    ref.set("qpos", 0, m, q[:m])
+    growth = {"plain": [], "mfma": []}
+    for t in range(0, 40 + steps, 20):
+        ref.step(0, m, 20)
+        for f, d in dev.items():
+            d.step(0, m, 20)
+            growth["mfma" if f else "plain"].append(float(np.abs(d.get("qpos") - ref.get("qpos")).max()))
+    out["max_abs_qpos_err_every_20_steps"] = growth
+    out["err_bound"] = ERR_BOUND
+    out["mfma_within_bound"] = bool(max(growth["mfma"]) <= ERR_BOUND)
     return out
 
 
@@ -327,7 +392,8 @@ def tumble_study(n: int, steps: int, seed: int, cpu_envs: int):
         dt = time.perf_counter() - t0
         kms, launches = sim.kernel_timing(reset=True)
         m = min(cpu_envs, n)
-        ref = OracleSim(arrays, m)
+        # the host runs the same solver (the model names Newton)
+        ref = OracleSim({**arrays, "solver_kind": 0 if solver == "newton" else 1}, m)
         plan.apply(ref, 0, m)
         c0 = time.perf_counter()
         ref.step(0, m, steps, threads=threads)
@@ -448,11 +514,17 @@ def main():
     ap.add_argument("--tumble", action="store_true",
                     help="only run the tabletop tumble study (8192 envs, 500 steps, both solvers, per-env object "
                          "sets with hull wedges, GPU and host oracle) and print one JSON line per solver")
+    ap.add_argument("--mfma-study", action="store_true",
+                    help="only run the study of Newton's Hessian on the FP64 matrix cores (6-body pile, 8192 envs): "
+                         "speed and distance from the oracle, one JSON line")
     ap.add_argument("--piles-general", action="store_true",
                     help="as --piles, on piles whose cylinders meet every body (general convex pairs)")
     args = ap.parse_args()
     if args.arm:
         print(json.dumps(arm_study(4096, 200, 0, 512)), flush=True)
+        return
+    if args.mfma_study:
+        print(json.dumps(mfma_study(8192, 200, 29, 256)), flush=True)
         return
     if args.tumble:
         for line in tumble_study(args.envs if args.envs != 16384 else 8192, 500, args.seed, 1024):

@@ -179,7 +179,13 @@ typedef struct ph_options {
   int32_t device;           /* HIP device ordinal */
   int32_t solver_override;  /* -1: keep the model's; else PH_SOLVER_* */
   int32_t max_contacts;     /* per-env contact capacity; 0 = derived from the model */
-  int32_t reserved[5];
+  /* 1: Newton's dense Hessian J'DJ (islands over 16 dofs or rows, at most 64
+   * dofs) on the FP64 matrix cores when one env owns a wavefront (launch shapes
+   * with 1 env per wavefront). No counterpart in the reference (newton.rs:279-300 sums
+   * (d_r J_ri) J_rj row by row); the fused accumulation is not bit-identical
+   * with it, so results then depend on the launch shape. 0 (default): off. */
+  int32_t hessian_mfma;
+  int32_t reserved[4];
 } ph_options;
 
 /* Text of the last error raised on the calling thread ("" if none). */
@@ -223,6 +229,13 @@ int pippin_hip_set(ph_sim* s, int32_t field, int64_t begin, int64_t end, const d
 /* Geom poses (envs x ngeom) and camera poses (envs x ncam). */
 int pippin_hip_poses(const ph_sim* s, int64_t begin, int64_t end, ph_pose* geoms, int64_t ngeoms_len,
                      ph_pose* cams, int64_t ncams_len);
+/* The same poses for every env in DEVICE memory, for a renderer on the GPU
+ * (the `Renderer` trait's consumer, crates/pippin-env/src/lib.rs:224-238; the
+ * host copy is pippin_cpu.rs:189-205): *geoms (envs x ngeom) and *cams
+ * (envs x ncam) point at buffers the sim owns until pippin_hip_destroy. Each
+ * call refreshes them to the current state, equal to pippin_hip_poses bit for
+ * bit, on the library stream, and returns after they are written. */
+int pippin_hip_poses_device(ph_sim* s, ph_pose** geoms, ph_pose** cams);
 
 /* ---- per-env parameters: Physics::{set,get}_param (lib.rs:130-141), params.rs:111-177 ---- */
 /* `values` holds (end - begin) x width doubles (width = Param::width_in). */

@@ -705,11 +705,37 @@ def _parse_mesh(ctx, bld, node, meshdir):
     bld.meshes.append((name, hull[0], hull[1]))
 
 
+def _expand_includes(node, directory: Path, depth: int):
+    """mjcf.rs:178-209: replace each <include file=.../> by the children of the
+    included document's root, files relative to the including file's folder,
+    at most 16 levels deep."""
+    kids = []
+    for k in list(node):
+        if k.tag != "include":
+            _expand_includes(k, directory, depth)
+            kids.append(k)
+            continue
+        path = directory / k.get("file", "")
+        try:
+            text = path.read_text()
+        except OSError as e:
+            raise MjcfError(f'<include file="{k.get("file", "")}">: {e}')
+        if depth + 1 > 16:  # the included text would be expanded one level deeper
+            raise MjcfError("<include> nested too deeply (cycle?)")
+        sub = ET.fromstring(text)
+        _expand_includes(sub, path.parent, depth + 1)
+        kids.extend(list(sub))
+    for k in list(node):
+        node.remove(k)
+    node.extend(kids)
+
+
 def load_string(xml: str, directory: str | Path = ".") -> dict:
-    """mjcf.rs:211-395 (without <include>)."""
+    """mjcf.rs:211-395."""
     root = ET.fromstring(xml)
     if root.tag != "mujoco":
         raise MjcfError("root element must be <mujoco>")
+    _expand_includes(root, Path(directory), 0)
     ctx = Ctx()
     meshdir = Path(directory)
     for c in root.findall("compiler"):

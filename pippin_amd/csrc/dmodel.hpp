@@ -96,6 +96,7 @@ struct DModel {
   int ncmax;  // contact capacity per env (the sum of per-pair maxima: never exceeded)
   int nrmax;  // row capacity per env (4 per contact for Newton, 3 for PGS, + limits + equalities)
   int nrl;    // rows kept in LDS; larger problems use the global row buffers
+  int hess_mfma;  // 1: Newton's dense Hessian J'DJ on the FP64 matrix cores (one env per wavefront only)
 
   gp<const int> body_parent, body_jntadr, body_jntnum, body_dofadr, body_dofnum, body_lastdof;
   gp<const uint64_t> body_chain;  // dofs on the path world..body (bit k = dof k)

@@ -12,6 +12,7 @@
 // dofs, pairs, matrix entries, Cholesky rows), so each lane performs exactly
 // the reference's sequence of operations for the values it owns.
 #pragma once
+#include <type_traits>
 #include "dgeneral.hpp"
 #include "lanes.hpp"
 
@@ -1112,6 +1113,57 @@ PHD int newton_lanes(const MT& M, Env& E, const NP& P) {
   return iters;
 }
 
+// Lanes an env owns in executor X (0 for executors without lane groups).
+template <class X, class = void>
+struct wave_lanes {
+  static constexpr int value = 0;
+};
+template <class X>
+struct wave_lanes<X, std::void_t<decltype(X::G)>> {
+  static constexpr int value = X::G;
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+// H = M + sum over active rows of (d_r J_r)' J_r on the FP64 matrix cores, for
+// one env owning the whole wavefront: 16 x 16 tiles of H, each accumulated by
+// v_mfma_f64_16x16x4f64 over the rows four at a time (inactive rows and the
+// padding beyond nv give zeros). Lane l feeds A[i = l % 16][k = l / 16] and
+// B[k = l / 16][j = l % 16] and holds H's column j = l % 16, rows l / 16,
+// l / 16 + 4, l / 16 + 8 and l / 16 + 12 of the tile. The fused multiply-adds and the summation order
+// differ from the reference's (d_r J_ri) J_rj added row by row, so results are
+// not bit-identical with it; used only when the sim asks for it.
+__device__ __noinline__ void hessian_mfma(const Real* qm, const Real* J, const Real* dd, const Real* act, int nr, int nv,
+                                          Real* H) {
+  const int l = (int)threadIdx.x, c16 = l & 15, q = l >> 4;
+  const int nt = (nv + 15) / 16;  // tiles per side (nv <= 64)
+  for (int ti = 0; ti < nt; ti++) {
+    f64x4 acc[4];
+    for (int tj = 0; tj < 4; tj++) acc[tj] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const int i = 16 * ti + c16;
+    for (int s = 0; s < nr; s += 4) {
+      const int r = s + q;
+      const bool on = r < nr && act[r] != 0.0;
+      const Real a = (on && i < nv) ? dd[r] * J[(int64_t)r * nv + i] : 0.0;
+      for (int tj = 0; tj < 4; tj++) {
+        if (tj >= nt) break;
+        const int j = 16 * tj + c16;
+        const Real b = (on && j < nv) ? J[(int64_t)r * nv + j] : 0.0;
+        acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[tj], 0, 0, 0);
+      }
+    }
+    // accumulator layout: lane l holds column l % 16, rows l / 16 + 4 t
+    for (int tj = 0; tj < nt; tj++) {
+      const int j = 16 * tj + c16;
+      for (int t = 0; t < 4; t++) {
+        const int ii = 16 * ti + q + 4 * t;
+        if (ii < nv && j < nv) H[ii * nv + j] = qm[ii * nv + j] + acc[tj][t];
+      }
+    }
+  }
+}
+#endif
+
 template <class X, class MT>
 PHD int newton_dense(const MT& M, Env& E, const NP& P) {
   const int nv = P.nv, nr = P.nr;
@@ -1203,8 +1255,18 @@ PHD int newton_dense(const MT& M, Env& E, const NP& P) {
     });
     if (isc[6]) break;
     X::par(nr, [&](int r) { prev[r] = act[r]; });
+    bool on_mfma = false;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (wave_lanes<X>::value == 64) {
+      if (M.hess_mfma && nv <= 64) {  // four 16-column tiles per row of tiles
+        hessian_mfma(qm, J, P.dd, act, nr, nv, H);
+        __syncthreads();
+        on_mfma = true;
+      }
+    }
+#endif
     // H = M + J_A' D J_A, entry-parallel, rows in order
-    X::par(nv * nv, [&](int t) {
+    if (!on_mfma) X::par(nv * nv, [&](int t) {
       int i = t / nv, j = t % nv;
       Real h = qm[t];
       if (i == j) h += 0.0;

@@ -896,11 +896,43 @@ void parse_mesh(Ctx& ctx, const XNode& node, const std::string& meshdir) {
   ctx.hulls.push_back(h);
 }
 
+// mjcf.rs:178-209: each <include file="..."/> is replaced, in place, by the
+// children of the included document's root element; files resolve against the
+// including file's folder; nesting is limited to 16 levels.
+void expand_includes(XNode& node, const std::string& dir, int depth) {
+  std::vector<std::unique_ptr<XNode>> kids;
+  for (auto& k : node.kids) {
+    if (k->tag != "include") {
+      expand_includes(*k, dir, depth);
+      kids.push_back(std::move(k));
+      continue;
+    }
+    const char* f = k->get("file");
+    std::string file = f ? f : "";
+    std::string path = (!file.empty() && file[0] == '/') ? file : dir + "/" + file;
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw LoadError{"<include file=\"" + file + "\">: cannot read " + path};
+    // the included text is expanded one level deeper; deeper than 16 is an error
+    if (depth + 1 > 16) throw LoadError{"<include> nested too deeply (cycle?)"};
+    std::stringstream ss;
+    ss << in.rdbuf();
+    const std::string text = ss.str();
+    XmlParser sub(text);
+    std::unique_ptr<XNode> sroot = sub.parse();
+    size_t slash = path.find_last_of('/');
+    std::string sub_dir = slash == std::string::npos ? dir : path.substr(0, slash);
+    expand_includes(*sroot, sub_dir, depth + 1);
+    for (auto& sk : sroot->kids) kids.push_back(std::move(sk));
+  }
+  node.kids = std::move(kids);
+}
+
 // mjcf.rs:211-395
 void load(const std::string& xml, const std::string& dir, phh::HostModel& m) {
   XmlParser parser(xml);
   std::unique_ptr<XNode> root = parser.parse();
   if (root->tag != "mujoco") throw LoadError{"root element must be <mujoco>"};
+  expand_includes(*root, dir.empty() ? "." : dir, 0);
   Ctx ctx;
   std::string meshdir = dir.empty() ? "." : dir;
   for (auto& c : root->kids) {

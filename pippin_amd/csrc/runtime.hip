@@ -52,6 +52,35 @@ __global__ void __launch_bounds__(64) k_kin(const DModel* __restrict__ Mp, const
   store_poses<DevX>(M, S, E);
 }
 
+// Poses as f32 `Pose` records (pippin-env/src/lib.rs:102-107) in device memory:
+// geoms from the double poses k_kin wrote, cameras from their body's pose
+// (pippin_cpu.rs:441-446), the same arithmetic as the host path of
+// pippin_hip_poses, then rounded to f32.
+__global__ void __launch_bounds__(256) k_pose_f32(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                                  const int* cam_body, const Real* cam_pos, const Real* cam_quat,
+                                                  ph_pose* geoms, ph_pose* cams) {
+  const DModel& M = *Mp;
+  const DState& S = *Sp;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ng = M.ngeom, nc = M.ncam, nb = M.nbody;
+  if (t < S.n * ng) {
+    ph_pose& p = geoms[t];
+    for (int k = 0; k < 3; k++) p.pos[k] = (float)S.geom_xpos[t * 3 + k];
+    for (int k = 0; k < 9; k++) p.mat[k] = (float)S.geom_xmat[t * 9 + k];
+  } else if (t < S.n * ng + S.n * nc) {
+    const int64_t u = t - S.n * ng, e = u / nc;
+    const int c = (int)(u % nc), b = cam_body[c];
+    const M3 xm = ldm(S.xmat + (e * nb + b) * 9);
+    const V3 pos = ld3(S.xpos + (e * nb + b) * 3) + mul_vec(xm, ld3(cam_pos + 3 * c));
+    const M3 mat = mul_mat(xm, to_mat(ldq(cam_quat + 4 * c)));
+    ph_pose& p = cams[u];
+    p.pos[0] = (float)pos.x;
+    p.pos[1] = (float)pos.y;
+    p.pos[2] = (float)pos.z;
+    for (int k = 0; k < 9; k++) p.mat[k] = (float)mat.m[k];
+  }
+}
+
 // Data::reset (data.rs:122-131) for listed envs: qacc is kept.
 __global__ void __launch_bounds__(64) k_reset(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
                                               const int64_t* ids) {
@@ -227,6 +256,10 @@ struct ph_sim {
   // envs on the first poses() call after qpos changed.
   mutable bool poses_stale = true;
   int64_t* reset_scratch = nullptr;  // this device's `scratch` (pippin_hip_reset_noalloc)
+  // f32 poses in device memory (pippin_hip_poses_device), allocated on first use
+  DBuf<ph_pose> d_pose_geom, d_pose_cam;
+  DBuf<int> d_cam_body;
+  DBuf<double> d_cam_pos, d_cam_quat;
   bool qpos_shared = false;  // a device qpos pointer was handed out (pippin_hip_state_device)
   hipStream_t stream = nullptr;
   DModel M;
@@ -393,6 +426,7 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
 
   DModel& M = s->M;
   phh::init_dmodel_scalars(hm, solver, opt ? opt->max_contacts : 0, M);
+  M.hess_mfma = opt && opt->hessian_mfma ? 1 : 0;
   if (M.L.bytes > 64 * 1024) return fail(PH_ERR_MODEL, "model too large for the per-env LDS working set");
   // Pack as many envs per workgroup as fit 64 KiB of LDS (PIPPIN_HIP_ENVS_PER_WG overrides: 1, 2 or 4,
   // up to the device's per-workgroup LDS).
@@ -1007,6 +1041,44 @@ extern "C" int pippin_hip_reset_noalloc(ph_sim* s, const int64_t* envs, int64_t 
   return PH_OK;
 }
 
+// Rerun kinematics for every env when qpos changed since the last poses (or
+// a device qpos pointer is out): PippinCpu's kinematics after step and set.
+static int refresh_poses(const ph_sim* s) {
+  if (!s->poses_stale && !s->qpos_shared) return PH_OK;
+  HIPCHK(hipSetDevice(s->device));
+  hipLaunchKernelGGL(k_kin, dim3((unsigned)s->n), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p,
+                     (int64_t)0, (const int64_t*)nullptr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->poses_stale = false;
+  return PH_OK;
+}
+
+extern "C" int pippin_hip_poses_device(ph_sim* s, ph_pose** geoms, ph_pose** cams) {
+  if (!s || !geoms || !cams) return fail(PH_ERR_ARG, "null argument");
+  const ph_model_desc& d = s->hm.d;
+  HIPCHK(hipSetDevice(s->device));
+  if (!s->d_pose_geom.p) {
+    HIPCHK(s->d_pose_geom.alloc((size_t)std::max<int64_t>(s->n * d.ngeom, 1)));
+    HIPCHK(s->d_pose_cam.alloc((size_t)std::max<int64_t>(s->n * d.ncam, 1)));
+    HIPCHK(s->d_cam_body.upload(s->hm.cam_body));
+    HIPCHK(s->d_cam_pos.upload(s->hm.cam_pos));
+    HIPCHK(s->d_cam_quat.upload(s->hm.cam_quat));
+  }
+  if (int rc = refresh_poses(s)) return rc;
+  const int64_t total = s->n * (d.ngeom + d.ncam);
+  if (total > 0) {
+    hipLaunchKernelGGL(k_pose_f32, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s->stream, s->d_model.p,
+                       s->d_state.p, s->d_cam_body.p, s->d_cam_pos.p, s->d_cam_quat.p, s->d_pose_geom.p,
+                       s->d_pose_cam.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  *geoms = s->d_pose_geom.p;
+  *cams = s->d_pose_cam.p;
+  return PH_OK;
+}
+
 extern "C" int pippin_hip_poses(const ph_sim* s, int64_t begin, int64_t end, ph_pose* geoms, int64_t ngl, ph_pose* cams,
                                 int64_t ncl) {
   if (int rc = check_range(s, begin, end)) return rc;
@@ -1015,14 +1087,7 @@ extern "C" int pippin_hip_poses(const ph_sim* s, int64_t begin, int64_t end, ph_
   if (int rc = check_len("poses(cams)", ncl, end - begin, d.ncam)) return rc;
   int64_t n = end - begin;
   if (n == 0) return PH_OK;
-  if (s->poses_stale || s->qpos_shared) {
-    HIPCHK(hipSetDevice(s->device));
-    hipLaunchKernelGGL(k_kin, dim3((unsigned)s->n), dim3(64), s->M.L.bytes, s->stream, s->d_model.p, s->d_state.p,
-                       (int64_t)0, (const int64_t*)nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s->stream));
-    s->poses_stale = false;
-  }
+  if (int rc = refresh_poses(s)) return rc;
   std::vector<double> gp(n * 3 * d.ngeom), gm(n * 9 * d.ngeom), bp(n * 3 * d.nbody), bm(n * 9 * d.nbody);
   if (d.ngeom) {
     HIPCHK(hipMemcpy(gp.data(), s->s_gxpos.p + begin * 3 * d.ngeom, gp.size() * 8, hipMemcpyDeviceToHost));

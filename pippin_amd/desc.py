@@ -111,7 +111,7 @@ class ph_launch_shape(C.Structure):
 
 class ph_options(C.Structure):
     _fields_ = [("device", C.c_int32), ("solver_override", C.c_int32), ("max_contacts", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("hessian_mfma", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 CONTACT_DTYPE = np.dtype([("pos", "<f8", 3), ("normal", "<f8", 3), ("tangent", "<f8", 3),

@@ -69,6 +69,7 @@ def lib():
             "pippin_hip_get": ([vp, i32, i64, i64, vp, i64], C.c_int),
             "pippin_hip_set": ([vp, i32, i64, i64, vp, i64], C.c_int),
             "pippin_hip_poses": ([vp, i64, i64, vp, i64, vp, i64], C.c_int),
+            "pippin_hip_poses_device": ([vp, C.POINTER(vp), C.POINTER(vp)], C.c_int),
             "pippin_hip_set_param": ([vp, i32, i64, i64, i64, vp, i64], C.c_int),
             "pippin_hip_get_param": ([vp, i32, i64, i64, i64, vp, i64], C.c_int),
             "pippin_hip_appearance": ([vp, i64, i64, vp, i64], C.c_int),
@@ -153,7 +154,7 @@ class HipPhysics:
     name = "pippin-hip"
 
     def __init__(self, model, num_envs: int, device: int = 0, solver: str | None = None,
-                 max_contacts: int = 0):
+                 max_contacts: int = 0, hessian_mfma: bool = False):
         if isinstance(model, (str, Path)):
             model = load_model(model)
         self.arrays = model
@@ -162,6 +163,7 @@ class HipPhysics:
         opt.device = device
         opt.solver_override = -1 if solver is None else SOLVER[solver]
         opt.max_contacts = max_contacts
+        opt.hessian_mfma = 1 if hessian_mfma else 0
         self.L = lib()
         self.h = C.c_void_p()
         rc = self.L.pippin_hip_create(self._desc.ptr(), int(num_envs), C.byref(opt), C.byref(self.h))
@@ -267,6 +269,13 @@ class HipPhysics:
         c = np.zeros((end - begin, self.ncam), dtype=POSE_DTYPE)
         self._check(self.L.pippin_hip_poses(self.h, begin, end, _p(g), g.size, _p(c), c.size))
         return g, c
+
+    def poses_device(self):
+        """Device addresses of the sim's f32 pose buffers (envs x ngeom, envs x ncam),
+        refreshed to the current state."""
+        g, c = C.c_void_p(), C.c_void_p()
+        self._check(self.L.pippin_hip_poses_device(self.h, C.byref(g), C.byref(c)))
+        return g.value, c.value
 
     def param_width(self, p):
         return {"geom_size": 3, "actuator_bias": 3, "body_inertia": 3, "body_ipos": 3,

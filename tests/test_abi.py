@@ -141,3 +141,57 @@ def test_degenerate_mesh_is_refused():
         physics.load_model(xml, is_xml=True)
     with pytest.raises(mjcf.MjcfError, match="degenerate"):
         mjcf.load_string(xml)
+
+
+def test_includes_splice_in_both_loaders(tmp_path):
+    """<include file=.../> (mjcf.rs:178-209): replaced by the children of the
+    included root, relative to the including file, recursively; the result is
+    the same model as the flat document, through both loaders; a cycle is an
+    error."""
+    (tmp_path / "parts").mkdir()
+    (tmp_path / "parts" / "legs.xml").write_text(
+        '<mujoco><body name="leg" pos="0.2 0 0"><joint axis="0 1 0"/>'
+        '<geom type="capsule" fromto="0 0 0 0.2 0 0" size="0.03"/></body></mujoco>')
+    (tmp_path / "parts" / "torso.xml").write_text(
+        '<mujoco><body name="torso" pos="0 0 0.5"><freejoint/><geom type="sphere" size="0.1"/>'
+        '<include file="legs.xml"/></body></mujoco>')
+    (tmp_path / "opts.xml").write_text('<mujoco><option timestep="0.005"/></mujoco>')
+    main = ('<mujoco><include file="opts.xml"/><worldbody><geom type="plane" size="5 5 .1"/>'
+            '<include file="parts/torso.xml"/></worldbody></mujoco>')
+    (tmp_path / "main.xml").write_text(main)
+    flat = ('<mujoco><option timestep="0.005"/><worldbody><geom type="plane" size="5 5 .1"/>'
+            '<body name="torso" pos="0 0 0.5"><freejoint/><geom type="sphere" size="0.1"/>'
+            '<body name="leg" pos="0.2 0 0"><joint axis="0 1 0"/>'
+            '<geom type="capsule" fromto="0 0 0 0.2 0 0" size="0.03"/></body></body></worldbody></mujoco>')
+    a = physics.load_model(tmp_path / "main.xml")
+    b = mjcf.load_file(tmp_path / "main.xml")
+    c = mjcf.load_string(flat)
+    assert a["timestep"] == 0.005 and int(a["nbody"]) == 3
+    for k in c:
+        vc = np.asarray(c[k]).reshape(-1)
+        for other in (a, b):
+            assert np.array_equal(np.asarray(other[k]).reshape(-1).astype(vc.dtype), vc), k
+    (tmp_path / "loop.xml").write_text('<mujoco><include file="loop.xml"/></mujoco>')
+    with pytest.raises(physics.HipError, match="nested too deeply"):
+        physics.load_model(tmp_path / "loop.xml")
+    with pytest.raises(mjcf.MjcfError, match="nested too deeply"):
+        mjcf.load_file(tmp_path / "loop.xml")
+
+
+@pytest.mark.parametrize("levels,ok", [(16, True), (17, False)])
+def test_include_depth_limit(tmp_path, levels, ok):
+    """mjcf.rs:179-183: included text is expanded up to 16 levels below the
+    top document; the 17th level is an error in both loaders."""
+    for k in range(1, levels + 1):
+        inner = f'<include file="l{k + 1}.xml"/>' if k < levels else '<geom type="plane" size="1 1 .1"/>'
+        (tmp_path / f"l{k}.xml").write_text(f"<mujoco>{inner}</mujoco>")
+    main = '<mujoco><worldbody><include file="l1.xml"/></worldbody></mujoco>'
+    (tmp_path / "main.xml").write_text(main)
+    if ok:
+        assert int(physics.load_model(tmp_path / "main.xml")["ngeom"]) == 1
+        assert int(mjcf.load_file(tmp_path / "main.xml")["ngeom"]) == 1
+    else:
+        with pytest.raises(physics.HipError, match="nested too deeply"):
+            physics.load_model(tmp_path / "main.xml")
+        with pytest.raises(mjcf.MjcfError, match="nested too deeply"):
+            mjcf.load_file(tmp_path / "main.xml")

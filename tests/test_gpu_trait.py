@@ -292,3 +292,49 @@ def test_set_device_is_ordered_before_get_set_and_step(assets):
     for s in (g, ref):
         s.step(0, n, 3)
     assert np.array_equal(g.get("qpos"), ref.get("qpos"))
+
+
+def test_device_poses_equal_host_poses(assets):
+    """pippin_hip_poses_device: the f32 Pose records of every env in device
+    memory, for a renderer on the GPU; equal to the host copy of poses() bit
+    for bit, after steps and after set(qpos)."""
+    from pippin_amd.desc import POSE_DTYPE
+    import ctypes
+
+    from pippin_amd import physics
+
+    # the Ant with a camera on its torso, so that camera poses move too
+    xml = (assets / "ant.xml").read_text()
+    i = xml.index(">", xml.index('<body name="torso"')) + 1
+    xml = xml[:i] + '<camera name="follow" pos="0 -1.5 0.6" quat="0.92 0.38 0 0"/>' + xml[i:]
+    arr = mjcf.load_string(xml)
+    assert int(arr["ncam"]) == 1
+    n = 40
+    g = HipPhysics(arr, n)
+    rng = np.random.default_rng(2)
+    hip = physics.lib()  # its HIP runtime (the one torch loaded) copies device to host
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemcpy.restype = ctypes.c_int
+
+    def device_copy(ptr, count):
+        out = np.zeros(count, dtype=POSE_DTYPE)
+        assert hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), out.nbytes, 2) == 0
+        return out
+
+    first = None
+    for t in range(3):
+        g.set("ctrl", 0, n, rng.uniform(-1, 1, size=(n, 8)))
+        g.step(0, n, 5)
+        if t == 2:
+            q = g.get("qpos")
+            q[:, 2] += 0.1
+            g.set("qpos", 0, n, q)
+        gp, cp = g.poses_device()
+        first = first or (gp, cp)
+        assert (gp, cp) == first  # the buffers stay put
+        hg, hc = g.poses()
+        dg = device_copy(gp, n * g.ngeom).reshape(n, g.ngeom)
+        assert np.array_equal(dg["pos"], hg["pos"]) and np.array_equal(dg["mat"], hg["mat"])
+        if g.ncam:
+            dc = device_copy(cp, n * g.ncam).reshape(n, g.ncam)
+            assert np.array_equal(dc["pos"], hc["pos"]) and np.array_equal(dc["mat"], hc["mat"])
