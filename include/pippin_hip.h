@@ -195,6 +195,19 @@ const char* pippin_hip_last_error(void);
 /* Parse an MJCF document (subset of mjcf.rs). `dir` resolves relative asset paths. */
 int pippin_hip_model_load_xml(const char* xml, const char* dir, ph_model** out);
 int pippin_hip_model_load_file(const char* path, ph_model** out);
+/* URDF import (replaces pippin::urdf::load_file, crates/pippin/src/urdf.rs:330-339,
+ * with UrdfOptions :22-28): the file is rewritten as MJCF (urdf.rs:205-327) and
+ * loaded as pippin_hip_model_load_xml loads it. floating_base != 0 gives the
+ * root link a free joint. package_dirs: ':'-separated directories searched for
+ * package://<pkg>/... URIs (NULL or "" for none); ROS_PACKAGE_PATH and the
+ * folders above the URDF file are searched too. Skipped geoms and unsupported
+ * joint types are reported on stderr, as the reference does. */
+int pippin_hip_model_load_urdf(const char* path, int32_t floating_base, const char* package_dirs, ph_model** out);
+/* The MJCF text a URDF document becomes (urdf::to_mjcf, urdf.rs:205). `base`
+ * resolves relative mesh paths. Writes the document and its terminating zero
+ * to out when cap allows (else PH_ERR_SIZE); *len receives its length. */
+int pippin_hip_urdf_to_mjcf(const char* urdf, const char* base, int32_t floating_base, const char* package_dirs,
+                            char* out, int64_t cap, int64_t* len);
 /* Borrow the flat descriptor of a loaded model (valid until pippin_hip_model_free). */
 const ph_model_desc* pippin_hip_model_desc(const ph_model* m);
 void pippin_hip_model_free(ph_model* m);

@@ -14,7 +14,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libpippin_hip.so"
-SOURCES = ["runtime.hip", "mlp.hip", "host_model.cpp", "mjcf.cpp", "meshes.cpp"]
+SOURCES = ["runtime.hip", "mlp.hip", "host_model.cpp", "mjcf.cpp", "meshes.cpp", "urdf.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PIPPIN_HIP_ARCH", "gfx950")
 FLAGS = [

@@ -1493,11 +1493,221 @@ PHD void solve_serial(const Real* L, int nv, int s0, int e, Real* x) {
 enum { PR_SEP = 0, PR_MU, PR_INVA, PR_LAM, PR_KIND, PR_NORMAL, PR_CONTACT };
 enum { PK_NORMAL = 0, PK_FRICTION = 1, PK_BILATERAL = 2 };
 
+// Soft-constraint coefficients and stopping rule of the sweeps (solver.rs:44-50, :229-240).
+struct PgsConsts {
+  Real h, slop, bias_rate, mass_scale, impulse_scale, max_bias, tol;
+  int iterations;
+};
+
+// One row's new force (solver.rs:175-207): jv = J_r v, lam = the row's force,
+// lam_n = the force of its normal row (friction rows).
+PHD Real pgs_row_force(const PgsConsts& C, int kind, Real inv_a, Real sep, Real mu, Real lam, Real lam_n, Real jv,
+                       bool use_bias) {
+  if (kind == PK_NORMAL) {
+    Real sp = sep + C.slop;
+    Real bias, ms, is;
+    if (sp > 0.0) {
+      bias = sp / C.h;
+      ms = 1.0;
+      is = 0.0;
+    } else if (use_bias) {
+      bias = rmax(C.bias_rate * sp, -C.max_bias);
+      ms = C.mass_scale;
+      is = C.impulse_scale;
+    } else {
+      bias = 0.0;
+      ms = 1.0;
+      is = 0.0;
+    }
+    Real delta = -inv_a * ms * (jv + bias) - is * lam;
+    return rmax(lam + delta, 0.0);
+  }
+  if (kind == PK_BILATERAL) {
+    Real bias, ms, is;
+    if (use_bias) {
+      bias = rclamp(C.bias_rate * sep, -C.max_bias, C.max_bias);
+      ms = C.mass_scale;
+      is = C.impulse_scale;
+    } else {
+      bias = 0.0;
+      ms = 1.0;
+      is = 0.0;
+    }
+    return lam - inv_a * ms * (jv + bias) - is * lam;
+  }
+  Real bound = mu * lam_n;
+  return rclamp(lam - inv_a * jv, -bound, bound);
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// x as held by lane l (l wave-uniform).
+__device__ __forceinline__ Real lane_read(Real x, int l) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// All Gauss-Seidel sweeps of one env that owns the whole wavefront (nv <= 64,
+// nr <= 64), with the env's state in registers: lane k holds v_k and the
+// J_rk and (M^-1 J^T)_rk of the next AHEAD rows, loaded ahead into buffers
+// that the rows take in turn (lanes past nv hold values nothing reads); lane r
+// holds row r's force, record and the terms of its update that stay fixed over
+// the sweeps (bias, mass and impulse scale of each phase, with the reference's
+// operations). Every lane computes each row's new force, so no barrier or LDS
+// round trip sits between rows. J_r v is summed in dof order over the dofs
+// where J_rk is nonzero: adding a zero product to a nonzero partial sum leaves
+// it unchanged, so this equals the reference's in-order sum over all dofs
+// unless the result is zero or not finite, or v holds a non-finite entry (a
+// zero J_rk times an infinite v_k is NaN); in those cases the full sum runs.
+// The sweep's largest velocity change max_r |dlambda_r| / inv_a_r is formed
+// after the sweep, one row per lane: each row changes at most once per sweep
+// and the maximum does not depend on the order. On return v, vpos and the
+// records' forces are written.
+//
+// The loads ahead are issued in one fixed order (J then M^-1 J^T, row after
+// row) on every path, each followed by a compiler barrier, so that the
+// compiler neither sinks them to their use nor waits for more than the row it
+// needs.
+#define PH_LOAD_FENCE() asm volatile("" ::: "memory")
+constexpr int AHEAD = 4;  // rows whose J and M^-1 J^T are in flight
+__device__ __noinline__ void pgs_sweeps_wave(const PgsConsts C, const Real* Jp, const Real* mjp, Real* rwp, int nr_in,
+                                             int nv_in, Real* vp, Real* vposp) {
+  // sizes and counts are the same in every lane: keep them scalar so that the
+  // loops are uniform
+  const int nr = __builtin_amdgcn_readfirstlane(nr_in), nv = __builtin_amdgcn_readfirstlane(nv_in);
+  const int iterations = __builtin_amdgcn_readfirstlane(C.iterations);
+  typedef const PH_AS_GLOBAL Real* GC;
+  typedef __attribute__((address_space(3))) Real* LP;
+  const GC J = (GC)Jp, mj = (GC)mjp;
+  PH_AS_GLOBAL Real* rw = (PH_AS_GLOBAL Real*)rwp;
+  const LP v = (LP)vp, vpos = (LP)vposp;
+  const int lane = (int)threadIdx.x;
+  const bool dof_lane = lane < nv;
+  const int kl = dof_lane ? lane : nv - 1;
+  Real vk = dof_lane ? v[lane] : 0.0;
+  Real inva = 0.0, mu = 0.0, lam = 0.0;
+  Real bias1 = 0.0, ms1 = 1.0, is1 = 0.0, bias0 = 0.0, ms0 = 1.0, is0 = 0.0;  // phase with / without bias
+  int kind = 0, nrm = 0;
+  if (lane < nr) {
+    const PH_AS_GLOBAL Real* q = rw + lane * 10;
+    inva = q[PR_INVA];
+    const Real sep = q[PR_SEP];
+    mu = q[PR_MU];
+    lam = q[PR_LAM];
+    kind = (int)q[PR_KIND];
+    nrm = (int)q[PR_NORMAL];
+    if (kind == PK_NORMAL) {
+      const Real sp = sep + C.slop;
+      if (sp > 0.0) {
+        bias1 = bias0 = sp / C.h;
+      } else {
+        bias1 = rmax(C.bias_rate * sp, -C.max_bias);
+        ms1 = C.mass_scale;
+        is1 = C.impulse_scale;
+      }
+    } else if (kind == PK_BILATERAL) {
+      bias1 = rclamp(C.bias_rate * sep, -C.max_bias, C.max_bias);
+      ms1 = C.mass_scale;
+      is1 = C.impulse_scale;
+    }
+  }
+  auto v_finite = [&]() { return __ballot(dof_lane && !__builtin_isfinite(vk)) == 0; };
+  bool vfin = v_finite();
+  auto sweep = [&](bool use_bias) {
+    Real dq = 0.0;  // |dlambda| of this lane's row, if it changed in this sweep
+    bool upd = false;
+    // row r with its J and M^-1 J^T entries in jb, mb; reloads them with row
+    // r + AHEAD. Every lane loads (lanes past nv repeat the last dof, rows past
+    // the end repeat the last row), so no branch sits around a load.
+    auto row = [&](int r, Real& jb, Real& mb) {
+      const int64_t off = (int64_t)(r + AHEAD < nr ? r + AHEAD : nr - 1) * nv + kl;
+      const Real inv_a = lane_read(inva, r);
+      const Real p = dof_lane ? jb * vk : 0.0;  // lanes past nv hold 0
+      const uint64_t nz = __ballot(dof_lane && jb != 0.0);
+      // both loads sit on the path of every row, skipped or not, so that the
+      // loads in flight are the same whichever way a row goes
+      jb = J[off];
+      PH_LOAD_FENCE();
+      if (inv_a != 0.0) {
+        Real jv = rsum0();
+        for (uint64_t m = nz; m; m &= m - 1) jv += lane_read(p, __builtin_ctzll(m));
+        if (!vfin || jv == 0.0 || !__builtin_isfinite(jv)) {
+          jv = rsum0();
+          for (int k = 0; k < nv; k++) jv += lane_read(p, k);
+        }
+        const Real lm = lane_read(lam, r);
+        const int kd = __builtin_amdgcn_readlane(kind, r);
+        Real nw;
+        if (kd == PK_FRICTION) {
+          const Real bound = lane_read(mu, r) * lane_read(lam, __builtin_amdgcn_readlane(nrm, r));
+          nw = rclamp(lm - inv_a * jv, -bound, bound);
+        } else {
+          const Real bias = lane_read(use_bias ? bias1 : bias0, r);
+          const Real ms = lane_read(use_bias ? ms1 : ms0, r);
+          const Real is = lane_read(use_bias ? is1 : is0, r);
+          if (kd == PK_NORMAL) {
+            const Real delta = -inv_a * ms * (jv + bias) - is * lm;
+            nw = rmax(lm + delta, 0.0);
+          } else {
+            nw = lm - inv_a * ms * (jv + bias) - is * lm;
+          }
+        }
+        const Real dl = nw - lm;
+        if (dl != 0.0) {
+          if (lane == r) {
+            lam = nw;
+            dq = fabs(dl);
+            upd = true;
+          }
+          vk += mb * dl;
+          vfin = v_finite();
+        }
+      }
+      mb = mj[off];
+      PH_LOAD_FENCE();
+    };
+    Real jbuf[AHEAD], mbuf[AHEAD];
+#pragma unroll
+    for (int b = 0; b < AHEAD; b++) {
+      const int64_t off = (int64_t)(b < nr ? b : nr - 1) * nv + kl;
+      jbuf[b] = J[off];
+      PH_LOAD_FENCE();
+      mbuf[b] = mj[off];
+      PH_LOAD_FENCE();
+    }
+    int r = 0;
+    for (; r + AHEAD <= nr; r += AHEAD) {  // every buffer on every pass: the loads in flight are alike at each pass
+#pragma unroll
+      for (int b = 0; b < AHEAD; b++) row(r + b, jbuf[b], mbuf[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < AHEAD - 1; b++)
+      if (r + b < nr) row(r + b, jbuf[b], mbuf[b]);
+    Real q = upd ? dq / inva : 0.0;
+    for (int o = 32; o >= 1; o >>= 1) q = rmax(q, __shfl_xor(q, o));
+    return q;
+  };
+  for (int it = 0; it < iterations; it++)
+    if (sweep(true) < C.tol) break;
+  if (dof_lane) vpos[lane] = vk;
+  const int relax = iterations / 3 > 2 ? iterations / 3 : 2;
+  for (int it = 0; it < relax; it++)
+    if (sweep(false) < C.tol) break;
+  if (dof_lane) v[lane] = vk;
+  if (lane < nr) rw[lane * 10 + PR_LAM] = lam;
+  __syncthreads();
+}
+#undef PH_LOAD_FENCE
+#endif
+
 // solver.rs:63-251. v: velocity after the smooth update (in, out); vpos:
 // the biased velocity the positions integrate with. Rows, M^-1 J^T and the
-// row records live in HBM; a Gauss-Seidel sweep is sequential over rows, so
-// one lane per env runs the sweeps while the rows, the M^-1 J^T columns and
-// the constraint force are formed lane-parallel.
+// row records live in HBM; the rows, the M^-1 J^T columns and the constraint
+// force are formed lane-parallel. A Gauss-Seidel sweep is sequential over rows:
+// when one env owns the wavefront it runs on registers (pgs_sweeps_wave),
+// otherwise one lane per env forms each row's force between lane-parallel
+// products and velocity updates.
 template <class X, class MT>
 PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
   const int nv = M.nv;
@@ -1654,8 +1864,16 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
   const Real a1 = 2.0 * M.zeta + h * omega;
   const Real a2 = h * omega * a1;
   const Real a3 = 1.0 / (1.0 + a2);
-  const Real bias_rate = omega / a1, mass_scale = a2 * a3, impulse_scale = a3;
-  const Real max_bias = 1.0, pgs_tol = 1e-5;
+  const PgsConsts C{h, M.slop, omega / a1, a2 * a3, a3, 1.0, 1e-5, M.iterations};
+  bool swept = false;
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (wave_lanes<X>::value == 64) {
+    if (nr <= 64) {
+      pgs_sweeps_wave(C, J, mj, rw, nr, nv, v, vpos);
+      swept = true;
+    }
+  }
+#endif
   // Gauss-Seidel sweeps (solver.rs:149-245): rows in order; for each row the
   // env's lanes form the products J_rk v_k and, after one lane has summed
   // them in dof order and updated the row's force, the update v += M^-1 J_r^T
@@ -1675,41 +1893,8 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
         for (int k = 0; k < nv; k++) jv += prod[k];
         const Real lam = q[PR_LAM];
         const int kind = (int)q[PR_KIND];
-        Real nw;
-        if (kind == PK_NORMAL) {
-          Real sp = q[PR_SEP] + M.slop;
-          Real bias, ms, is;
-          if (sp > 0.0) {
-            bias = sp / h;
-            ms = 1.0;
-            is = 0.0;
-          } else if (use_bias) {
-            bias = rmax(bias_rate * sp, -max_bias);
-            ms = mass_scale;
-            is = impulse_scale;
-          } else {
-            bias = 0.0;
-            ms = 1.0;
-            is = 0.0;
-          }
-          Real delta = -inv_a * ms * (jv + bias) - is * lam;
-          nw = rmax(lam + delta, 0.0);
-        } else if (kind == PK_BILATERAL) {
-          Real bias, ms, is;
-          if (use_bias) {
-            bias = rclamp(bias_rate * q[PR_SEP], -max_bias, max_bias);
-            ms = mass_scale;
-            is = impulse_scale;
-          } else {
-            bias = 0.0;
-            ms = 1.0;
-            is = 0.0;
-          }
-          nw = lam - inv_a * ms * (jv + bias) - is * lam;
-        } else {
-          Real bound = q[PR_MU] * rw[(int)q[PR_NORMAL] * 10 + PR_LAM];
-          nw = rclamp(lam - inv_a * jv, -bound, bound);
-        }
+        const Real lam_n = kind == PK_FRICTION ? rw[(int)q[PR_NORMAL] * 10 + PR_LAM] : 0.0;
+        const Real nw = pgs_row_force(C, kind, inv_a, q[PR_SEP], q[PR_MU], lam, lam_n, jv, use_bias);
         const Real dl = nw - lam;
         sc[9] = dl;
         if (dl != 0.0) {
@@ -1725,12 +1910,14 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
     }
     return sc[8];
   };
-  for (int it = 0; it < M.iterations; it++)
-    if (iterate(true) < pgs_tol) break;
-  X::par(nv, [&](int k) { vpos[k] = v[k]; });
-  const int relax = M.iterations / 3 > 2 ? M.iterations / 3 : 2;
-  for (int it = 0; it < relax; it++)
-    if (iterate(false) < pgs_tol) break;
+  if (!swept) {
+    for (int it = 0; it < M.iterations; it++)
+      if (iterate(true) < C.tol) break;
+    X::par(nv, [&](int k) { vpos[k] = v[k]; });
+    const int relax = M.iterations / 3 > 2 ? M.iterations / 3 : 2;
+    for (int it = 0; it < relax; it++)
+      if (iterate(false) < C.tol) break;
+  }
   // qfrc_constraint = sum_r J_r lambda_r / h, rows in order
   X::par(nv, [&](int k) {
     Real f = 0.0;

@@ -57,6 +57,9 @@ def lib():
         sigs = {
             "pippin_hip_model_load_xml": ([C.c_char_p, C.c_char_p, C.POINTER(vp)], C.c_int),
             "pippin_hip_model_load_file": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+            "pippin_hip_model_load_urdf": ([C.c_char_p, i32, C.c_char_p, C.POINTER(vp)], C.c_int),
+            "pippin_hip_urdf_to_mjcf": ([C.c_char_p, C.c_char_p, i32, C.c_char_p, C.c_char_p, i64, C.POINTER(i64)],
+                                        C.c_int),
             "pippin_hip_model_desc": ([vp], C.POINTER(ph_model_desc)),
             "pippin_hip_model_free": ([vp], None),
             "pippin_hip_create": ([vp, i64, vp, C.POINTER(vp)], C.c_int),
@@ -137,6 +140,35 @@ def load_model(path_or_xml: str | Path, is_xml: bool = False) -> dict:
         return arrays_from_desc(L.pippin_hip_model_desc(h).contents)
     finally:
         L.pippin_hip_model_free(h)
+
+
+def load_urdf(path: str | Path, floating_base: bool = False, package_dirs=()) -> dict:
+    """Import a URDF file with the library's loader (pippin::urdf::load_file,
+    urdf.rs:330-339): the raw descriptor as numpy arrays."""
+    L = lib()
+    h = C.c_void_p()
+    dirs = ":".join(str(d) for d in package_dirs).encode()
+    rc = L.pippin_hip_model_load_urdf(str(path).encode(), int(bool(floating_base)), dirs, C.byref(h))
+    if rc != 0:
+        raise HipError(_err())
+    try:
+        return arrays_from_desc(L.pippin_hip_model_desc(h).contents)
+    finally:
+        L.pippin_hip_model_free(h)
+
+
+def urdf_to_mjcf(urdf: str, base: str | Path = ".", floating_base: bool = False, package_dirs=()) -> str:
+    """The MJCF text the library makes of a URDF document (urdf::to_mjcf, urdf.rs:205)."""
+    L = lib()
+    n = C.c_int64(0)
+    args = (urdf.encode(), str(base).encode(), int(bool(floating_base)), ":".join(str(d) for d in package_dirs).encode())
+    rc = L.pippin_hip_urdf_to_mjcf(*args, None, 0, C.byref(n))
+    if rc not in (0, -2) or (rc == -2 and n.value <= 0):
+        raise HipError(_err())
+    buf = C.create_string_buffer(n.value + 1)
+    if L.pippin_hip_urdf_to_mjcf(*args, buf, n.value + 1, C.byref(n)) != 0:
+        raise HipError(_err())
+    return buf.value.decode()
 
 
 def selftest_math(x, device: int = 0):

@@ -81,6 +81,30 @@ def arm_gripper_xml(seed: int) -> str:
             '</actuator></mujoco>')
 
 
+def arm_env_params(arrays: dict, n: int, seed: int):
+    """Per-env draws for the arm configuration: link masses U[0.5, 3] kg for the
+    seven links (body_mass scales each link's inertia with it, params.rs:145-155),
+    cube half-extent U[0.02, 0.03] m with the cube resting on the table
+    (qpos0), cube friction U[0.6, 1.0]. Returns ([(param, id, values (n, width))],
+    qpos0 (n, nq)); apply the params, then qpos0, then reset."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed + 7919)
+    nbody = int(arrays["nbody"])
+    cube_geom, cube_body = int(arrays["ngeom"]) - 1, nbody - 1
+    links = range(2, 9)  # l0..l6 after the world and the massless base
+    mass = rng.uniform(0.5, 3.0, (n, 7))
+    half = rng.uniform(0.02, 0.03, n)
+    fric = rng.uniform(0.6, 1.0, n)
+    params = [("body_mass", b, mass[:, i:i + 1]) for i, b in enumerate(links)]
+    params.append(("geom_size", cube_geom, np.stack([half, half, half], axis=1)))
+    params.append(("geom_friction", cube_geom, fric[:, None]))
+    q0 = np.tile(np.asarray(arrays["qpos0"], dtype=np.float64), (n, 1))
+    za = int(np.asarray(arrays["jnt_qposadr"])[int(np.asarray(arrays["body_jntadr"])[cube_body])]) + 2
+    q0[:, za] = 0.2 + half  # table top at 0.2
+    return params, q0
+
+
 def arm_targets(seed: int, n: int, steps: int, nu: int, every: int = 50):
     """Joint-space targets along random smooth waypoints (cosine blends
     between waypoints drawn every `every` steps), per env; the last actuator

@@ -464,3 +464,31 @@ def test_tabletop_tumble_exact(solver):
             assert np.array_equal(o.get("qpos"), e.get("qpos")), (env, t)
             total += int(o.ncon().sum())
     assert total > 0
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_urdf_arm_exact(tmp_path, solver):
+    """The URDF arm (oracle URDF import) through the kernel body on the host:
+    positions and contact counts equal the oracle's over 150 steps of a fall
+    onto the floor."""
+    from oracle import urdf
+    from tests.test_urdf import ARM
+
+    path = tmp_path / "arm.urdf"
+    path.write_text(ARM)
+    arr = urdf.load_file(path)
+    arr["solver_kind"] = solver
+    n = 4
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(11)
+    q = o.get("qpos") + rng.normal(size=(n, int(arr["nq"]))) * 0.3
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+    seen = 0
+    for t in range(150):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+        seen += int(o.ncon().sum())
+    assert seen > 0

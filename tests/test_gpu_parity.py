@@ -394,24 +394,23 @@ def test_random_piles_with_general_pairs_match(seed, solver):
 
 
 @pytest.mark.parametrize("solver", [0, 1])
-def test_arm_gripper_with_per_env_cubes_match(solver):
-    """The 7-DoF arm with tendon-coupled gripper, cube size and friction
-    varied per env group (copy-on-write parameter variants), 200 steps on
-    smooth random targets; Newton and PGS."""
-    from pippin_amd.workloads import arm_gripper_xml, arm_targets
+def test_arm_gripper_with_per_env_draws_match(solver):
+    """The 7-DoF arm with tendon-coupled gripper, link masses, cube size,
+    resting height and friction drawn per env (copy-on-write parameter
+    variants, one per env), 200 steps on smooth random targets; Newton and PGS."""
+    from pippin_amd.workloads import arm_env_params, arm_gripper_xml, arm_targets
 
     arr = load_model(arm_gripper_xml(5), is_xml=True)  # the library's own loader
     arr["solver_kind"] = solver
     n = 24
     o, g = OracleSim(arr, n), HipPhysics(arr, n)
-    cube = int(arr["ngeom"]) - 1
-    rng = np.random.default_rng(5)
-    for b in range(0, n, 6):
-        h = rng.uniform(0.02, 0.03)
-        f = rng.uniform(0.6, 1.0)
-        for s in (o, g):
-            s.set_param("geom_size", cube, b, b + 6, [[h, h, h]] * 6)
-            s.set_param("geom_friction", cube, b, b + 6, [[f]] * 6)
+    params, qpos0 = arm_env_params(arr, n, 5)
+    for s in (o, g):
+        for name, gid, values in params:
+            s.set_param(name, gid, 0, n, np.ascontiguousarray(values))
+        s.set_param("qpos0", 0, 0, n, qpos0)
+        s.reset(np.arange(n))
+    assert np.array_equal(o.get("qpos"), g.get("qpos"))
     tg = arm_targets(5, n, 200, int(arr["nu"]))
     say("arm gripper", n)
     for t in range(200):
@@ -471,3 +470,33 @@ def test_tabletop_tumble_match(solver):
         assert np.array_equal(o.ncon(), g.ncon()), t
         assert np.array_equal(o.get("qpos"), g.get("qpos")), t
     assert o.ncon().sum() > 0
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_urdf_arm_matches(tmp_path, solver):
+    """A URDF arm (revolute, continuous, prismatic and fixed joints, capsule,
+    cylinder, sphere and box links over a box floor) imported by the library's
+    URDF loader and stepped 300 steps from a perturbed pose falling onto the
+    floor: positions and contact counts equal the oracle's at every step."""
+    from pippin_amd.physics import load_urdf
+    from tests.test_urdf import ARM
+
+    path = tmp_path / "arm.urdf"
+    path.write_text(ARM)
+    arr = load_urdf(path)
+    arr["solver_kind"] = solver
+    n = 16
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    rng = np.random.default_rng(11)
+    q = o.get("qpos") + rng.normal(size=(n, int(arr["nq"]))) * 0.3
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    say("urdf arm", n, "solver", solver)
+    seen = 0
+    for t in range(300):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+        seen += int(o.ncon().sum())
+    assert seen > 0
