@@ -141,28 +141,35 @@ def copy_calibration(gib: float) -> None:
     print(json.dumps({"copy_calibration_bytes_each_way": n * 8, "copies": 5}), flush=True)
 
 
-def warm_start_study(model: str, n: int, sub: int, record: int, train_iters: int, measure: int, seed: int):
+def warm_start_study(model: str, n: int, sub: int, record: int, train_iters: int, measure: int, seed: int,
+                     arrays=None, prepare=None, sim_kw=None):
     """Mean Newton iterations per physics step with the reference warm start and
     with the learned candidate (PH_MLP_WARM), from the same states, and the step
     kernel time of each. The network is trained online on contacts recorded by
-    the same sim (PH_MLP_LOG) before the measurement."""
+    the same sim (PH_MLP_LOG) before the measurement. `model` names an asset
+    file, or labels `arrays` (a loaded model) whose sims `prepare(sim)` sets up."""
     import torch
     from pippin_amd.physics import HipPhysics, load_model
 
-    arrays = load_model(ROOT / "pippin_amd" / "assets" / model)
-    ref, mlp = HipPhysics(arrays, n), HipPhysics(arrays, n)
+    if arrays is None:
+        arrays = load_model(ROOT / "pippin_amd" / "assets" / model)
+    ref, mlp = HipPhysics(arrays, n, **(sim_kw or {})), HipPhysics(arrays, n, **(sim_kw or {}))
     mlp.mlp_init(hidden=64, seed=seed, max_batch=8192)
     rng = np.random.default_rng(seed)
-    q = ref.get("qpos")
-    nfree = [j for j in range(int(arrays["njnt"])) if int(np.asarray(arrays["jnt_type"])[j]) == 0]
-    if nfree and model == "ant.xml":
-        q = initial_qpos(np.asarray(arrays["qpos0"]), n, seed)
-    else:  # move each free body in the plane only
-        adr = np.asarray(arrays["jnt_qposadr"])
-        for j in nfree:
-            q[:, adr[j]:adr[j] + 2] += rng.normal(0.0, 0.01, size=(n, 2))
-    for s in (ref, mlp):
-        s.set("qpos", 0, n, q)
+    if prepare is not None:
+        for s in (ref, mlp):
+            prepare(s)
+    else:
+        q = ref.get("qpos")
+        nfree = [j for j in range(int(arrays["njnt"])) if int(np.asarray(arrays["jnt_type"])[j]) == 0]
+        if nfree and model == "ant.xml":
+            q = initial_qpos(np.asarray(arrays["qpos0"]), n, seed)
+        else:  # move each free body in the plane only
+            adr = np.asarray(arrays["jnt_qposadr"])
+            for j in nfree:
+                q[:, adr[j]:adr[j] + 2] += rng.normal(0.0, 0.01, size=(n, 2))
+        for s in (ref, mlp):
+            s.set("qpos", 0, n, q)
     ctrl = rng.uniform(-1.0, 1.0, size=(record + measure, n, max(ref.nu, 1)))
     mlp.mlp_mode(HipPhysics.MLP_LOG)
     for a in range(record):
@@ -203,7 +210,7 @@ def warm_start_study(model: str, n: int, sub: int, record: int, train_iters: int
             "kernel_ms_per_step_reference": kr / lr_, "kernel_ms_per_step_learned": km / lm}
 
 
-def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = False):
+def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = False, solvers=("newton", "pgs")):
     """Tabletop piles (pippin_amd/workloads.py random_pile_xml: 2 to 6 free
     primitives dropped onto a plane), after the reference's tumble scenes:
     env-steps/s on the GPU (4 physics steps per launch, bodies settling and
@@ -228,7 +235,7 @@ def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = Fal
             failures.append(f"{where}: {ex}")
     arrays = load_model(random_pile_xml(seed, general), is_xml=True)
     out = []
-    for solver in ("newton", "pgs"):
+    for solver in solvers:
         sim = HipPhysics(arrays, n, solver=solver)
         rng = np.random.default_rng(seed)
         q = sim.get("qpos")
@@ -253,19 +260,21 @@ def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = Fal
         dt = time.perf_counter() - t0
         kms, launches = sim.kernel_timing(reset=True)
         m = min(cpu_envs, n)
-        # the host runs the same solver (the model file names Newton)
-        ref = OracleSim({**arrays, "solver_kind": 0 if solver == "newton" else 1}, m)
-        ref.set("qpos", 0, m, q[:m])
-        ref.step(0, m, 40 + 4 * tuned, threads=threads)
-        c0 = time.perf_counter()
-        ref.step(0, m, steps, threads=threads)
-        cdt = time.perf_counter() - c0
-        same = {"envs": m, "qpos": bool(np.array_equal(ref.get("qpos"), sim.get("qpos", 0, m))),
-                "ncon": bool(np.array_equal(ref.ncon(), sim.ncon(0, m)))}
+        cdt, same = None, None
+        if m > 0:
+            # the host runs the same solver (the model file names Newton)
+            ref = OracleSim({**arrays, "solver_kind": 0 if solver == "newton" else 1}, m)
+            ref.set("qpos", 0, m, q[:m])
+            ref.step(0, m, 40 + 4 * tuned, threads=threads)
+            c0 = time.perf_counter()
+            ref.step(0, m, steps, threads=threads)
+            cdt = time.perf_counter() - c0
+            same = {"envs": m, "qpos": bool(np.array_equal(ref.get("qpos"), sim.get("qpos", 0, m))),
+                    "ncon": bool(np.array_equal(ref.ncon(), sim.ncon(0, m)))}
         kind = ", cylinders meet every body" if general else ""
         out.append({"workload": f"pile seed {seed} ({nb} bodies{kind}), {solver}", "envs": n, "steps": steps,
                     "gpu_env_steps_per_s": n * steps / dt, "kernel_ms_per_launch": kms / launches,
-                    "cpu_env_steps_per_s": m * steps / cdt, "cpu_envs": m, "cpu_threads": threads,
+                    "cpu_env_steps_per_s": m * steps / cdt if cdt else None, "cpu_envs": m, "cpu_threads": threads,
                     "equal_to_oracle": same,
                     "newton_iters_mean": float(sim.solver_iterations().mean()),
                     "launch_shape": sim.launch_shape(),
@@ -402,7 +411,7 @@ def tumble_study(n: int, steps: int, seed: int, cpu_envs: int):
         same_n = bool(np.array_equal(ref.ncon(), sim.ncon(0, m)))
         out.append({"workload": f"tabletop tumble (seed {seed}), {solver}", "envs": n, "steps": steps,
                     "gpu_env_steps_per_s": n * steps / dt, "kernel_ms_per_launch": kms / launches,
-                    "cpu_env_steps_per_s": m * steps / cdt, "cpu_envs": m, "cpu_threads": threads,
+                    "cpu_env_steps_per_s": m * steps / cdt if cdt else None, "cpu_envs": m, "cpu_threads": threads,
                     "equal_to_oracle": {"envs": m, "qpos": same_q, "ncon": same_n},
                     "objects_per_env_mean": float(plan.counts.mean()),
                     "wedges": int((plan.kinds == 4).sum()),
@@ -495,6 +504,10 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device-to-device copy of this many GiB five times "
                          "(reference kernel for scaling HBM byte counters), then exit")
+    ap.add_argument("--pile-gpu", default="",
+                    help="SEED,SOLVER: one pile study (8192 envs, 200 steps) on the GPU only, for profiling")
+    ap.add_argument("--warm-start-wide", action="store_true",
+                    help="the learned warm-start study on the six-body pile and the tabletop tumble (Newton)")
     ap.add_argument("--warm-start", action="store_true",
                     help="only run the learned warm-start study (Ant and box_drop: mean Newton iterations "
                          "with and without the candidate) and print one JSON line per model")
@@ -527,6 +540,11 @@ def main():
         for line in tumble_study(args.envs if args.envs != 16384 else 8192, 500, args.seed, 1024):
             print(json.dumps(line), flush=True)
         return
+    if args.pile_gpu:
+        seed, solver = args.pile_gpu.split(",")
+        for line in pile_study(8192, 200, int(seed), 0, solvers=(solver,)):
+            print(json.dumps(line), flush=True)
+        return
     if args.piles or args.piles_general:
         seeds = (13, 18) if args.piles_general else (11, 13, 29)
         for seed in seeds:
@@ -538,6 +556,20 @@ def main():
                                           ("box_drop.xml", args.envs, 1, 400, 100)):
             log(f"warm-start study: {model}, {n} envs")
             print(json.dumps(warm_start_study(model, n, sub, rec, 1500, meas, args.seed)), flush=True)
+        return
+    if args.warm_start_wide:
+        # the wide Newton workloads: the six-body pile and the tabletop tumble, 8192 envs
+        from pippin_amd.physics import load_model
+        from pippin_amd.workloads import random_pile_xml, tabletop_tumble
+        pile = load_model(random_pile_xml(29), is_xml=True)
+        log("warm-start study: pile seed 29, 8192 envs")
+        print(json.dumps(warm_start_study("pile seed 29 (6 bodies, nv 36)", 8192, 1, 200, 1500, 100, args.seed,
+                                          arrays=pile)), flush=True)
+        plan = tabletop_tumble(8192, args.seed)
+        log("warm-start study: tabletop tumble, 8192 envs")
+        print(json.dumps(warm_start_study("tabletop tumble (seed 0)", 8192, 1, 200, 1500, 100, args.seed,
+                                          arrays=load_model(plan.xml, is_xml=True), prepare=plan.apply,
+                                          sim_kw={"max_contacts": 96})), flush=True)
         return
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)

@@ -172,4 +172,6 @@ def test_online_training_lowers_loss_and_iterations(assets):
     a, b = np.mean(it_ref), np.mean(it_mlp)
     say(f"mean Newton iterations: reference warm start {a:.3f}, with the learned candidate {b:.3f}"
         f" ({100 * (a - b) / a:+.1f} %)")
-    assert b <= a * 1.02
+    # training and stepping are deterministic, so the reduction is a measured
+    # constant of this setup (2.081 -> 2.031 iterations, 2.4 %); it must not vanish
+    assert (a - b) / a >= 0.01, (a, b)
