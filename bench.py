@@ -245,10 +245,11 @@ def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = Fal
         sim.set("qpos", 0, n, q)
         failures.clear()
         step(sim, 40, "drop")
-        # autotune the launch shape on the settling pile (untimed; results do not depend on it)
-        sim.autotune(rounds=1)
+        # autotune the launch shape on the settling pile (untimed; results do not depend on it);
+        # three timed rounds, so that each shape's median spans the settling
+        sim.autotune(rounds=3)
         tuned = 0
-        while sim.autotune_pending() and tuned < 64:
+        while sim.autotune_pending() and tuned < 200:
             step(sim, 4, f"autotune launch {tuned}")
             tuned += 1
         torch.cuda.synchronize()
@@ -380,9 +381,9 @@ def tumble_study(n: int, steps: int, seed: int, cpu_envs: int):
 
         tuner = HipPhysics(arrays, n, solver=solver, max_contacts=96)
         plan.apply(tuner)
-        tuner.autotune(rounds=1, cache_path=cache)
+        tuner.autotune(rounds=3, cache_path=cache)
         tuned = 0
-        while tuner.autotune_pending() and tuned < 64:
+        while tuner.autotune_pending() and tuned < 200:
             step(tuner, 4, f"autotune launch {tuned}")
             tuned += 1
         tuner.close()

@@ -324,7 +324,10 @@ void* pippin_hip_stream(ph_sim* s);
 enum {
   PH_SCHED_GRID = 0,      /* one workgroup per K envs, dispatched by the hardware */
   PH_SCHED_WS = 1,        /* persistent workgroups taking K envs at a time from a counter */
-  PH_SCHED_WS_BUCKET = 2  /* as WS, envs ordered by Newton iterations of their last launch */
+  PH_SCHED_WS_BUCKET = 2, /* as WS, envs ordered by Newton iterations of their last launch */
+  PH_SCHED_PGS_SPLIT = 3  /* PGS models, K = 1, generic kernel: each physics step as three launches
+                             (up to the sweeps, the sweeps with one env per wavefront and no LDS,
+                             the rest), so that many sweep wavefronts share a SIMD */
 };
 typedef struct ph_launch_shape {
   int32_t envs_per_wave; /* K: 1, 2 or 4 envs per 64-lane wavefront */

@@ -1549,21 +1549,23 @@ __device__ __forceinline__ Real lane_read(Real x, int l) {
 }
 
 // All Gauss-Seidel sweeps of one env that owns the whole wavefront (nv <= 64,
-// nr <= 64), with the env's state in registers: lane k holds v_k and the
+// nr <= 128), with the env's state in registers: lane k holds v_k and the
 // J_rk and (M^-1 J^T)_rk of the next AHEAD rows, loaded ahead into buffers
-// that the rows take in turn (lanes past nv hold values nothing reads); lane r
-// holds row r's force, record and the terms of its update that stay fixed over
-// the sweeps (bias, mass and impulse scale of each phase, with the reference's
-// operations). Every lane computes each row's new force, so no barrier or LDS
-// round trip sits between rows. J_r v is summed in dof order over the dofs
-// where J_rk is nonzero: adding a zero product to a nonzero partial sum leaves
-// it unchanged, so this equals the reference's in-order sum over all dofs
-// unless the result is zero or not finite, or v holds a non-finite entry (a
-// zero J_rk times an infinite v_k is NaN); in those cases the full sum runs.
-// The sweep's largest velocity change max_r |dlambda_r| / inv_a_r is formed
-// after the sweep, one row per lane: each row changes at most once per sweep
-// and the maximum does not depend on the order. On return v, vpos and the
-// records' forces are written.
+// that the rows take in turn (lanes past nv hold values nothing reads); lane l
+// holds rows l and l + 64: their forces, records and the terms of their
+// updates that stay fixed over the sweeps (bias, mass and impulse scale of
+// each phase, with the reference's operations). Every lane computes each
+// row's new force, so no barrier or LDS round trip sits between rows. J_r v is
+// summed in dof order over the dofs where J_rk is nonzero: adding a zero
+// product to a nonzero partial sum leaves it unchanged, so this equals the
+// reference's in-order sum over all dofs unless the result is zero or not
+// finite, or v holds a non-finite entry (a zero J_rk times an infinite v_k is
+// NaN); in those cases the full sum runs. The sweep's largest velocity change
+// max_r |dlambda_r| / inv_a_r is formed after the sweep, from each lane's
+// rows: each row changes at most once per sweep and the maximum does not
+// depend on the order. Each lane passes its entry
+// of v and receives its entries of v and vpos; the records' forces are
+// written before the return.
 //
 // The loads ahead are issued in one fixed order (J then M^-1 J^T, row after
 // row) on every path, each followed by a compiler barrier, so that the
@@ -1571,58 +1573,70 @@ __device__ __forceinline__ Real lane_read(Real x, int l) {
 // needs.
 #define PH_LOAD_FENCE() asm volatile("" ::: "memory")
 constexpr int AHEAD = 4;  // rows whose J and M^-1 J^T are in flight
-__device__ __noinline__ void pgs_sweeps_wave(const PgsConsts C, const Real* Jp, const Real* mjp, Real* rwp, int nr_in,
-                                             int nv_in, Real* vp, Real* vposp) {
+struct LaneVel {
+  Real v, vpos;  // this lane's entry of the solved velocity and of the one positions integrate with
+};
+constexpr int PGS_WAVE_ROWS = 128;  // rows the register sweeps hold: two per lane
+__device__ __forceinline__ LaneVel pgs_sweeps_body(const PgsConsts C, const Real* Jp, const Real* mjp, Real* rwp,
+                                                   int nr_in, int nv_in, Real v_in) {
   // sizes and counts are the same in every lane: keep them scalar so that the
   // loops are uniform
   const int nr = __builtin_amdgcn_readfirstlane(nr_in), nv = __builtin_amdgcn_readfirstlane(nv_in);
   const int iterations = __builtin_amdgcn_readfirstlane(C.iterations);
   typedef const PH_AS_GLOBAL Real* GC;
-  typedef __attribute__((address_space(3))) Real* LP;
   const GC J = (GC)Jp, mj = (GC)mjp;
   PH_AS_GLOBAL Real* rw = (PH_AS_GLOBAL Real*)rwp;
-  const LP v = (LP)vp, vpos = (LP)vposp;
   const int lane = (int)threadIdx.x;
   const bool dof_lane = lane < nv;
   const int kl = dof_lane ? lane : nv - 1;
-  Real vk = dof_lane ? v[lane] : 0.0;
-  Real inva = 0.0, mu = 0.0, lam = 0.0;
-  Real bias1 = 0.0, ms1 = 1.0, is1 = 0.0, bias0 = 0.0, ms0 = 1.0, is0 = 0.0;  // phase with / without bias
-  int kind = 0, nrm = 0;
-  if (lane < nr) {
-    const PH_AS_GLOBAL Real* q = rw + lane * 10;
-    inva = q[PR_INVA];
+  Real vk = dof_lane ? v_in : 0.0;
+  // row lane + 64 s in slot s: force, record, and the bias and scales of the
+  // phase with bias (the phase without has the bias bias0, scale 1, impulse 0)
+  Real inva[2] = {0.0, 0.0}, mu[2] = {0.0, 0.0}, lam[2] = {0.0, 0.0};
+  Real bias1[2] = {0.0, 0.0}, ms1[2] = {1.0, 1.0}, is1[2] = {0.0, 0.0}, bias0[2] = {0.0, 0.0};
+  int kind[2] = {0, 0}, nrm[2] = {0, 0};
+#pragma unroll
+  for (int sl = 0; sl < 2; sl++) {
+    const int row_id = lane + 64 * sl;
+    if (row_id >= nr) continue;
+    const PH_AS_GLOBAL Real* q = rw + row_id * 10;
+    inva[sl] = q[PR_INVA];
     const Real sep = q[PR_SEP];
-    mu = q[PR_MU];
-    lam = q[PR_LAM];
-    kind = (int)q[PR_KIND];
-    nrm = (int)q[PR_NORMAL];
-    if (kind == PK_NORMAL) {
+    mu[sl] = q[PR_MU];
+    lam[sl] = q[PR_LAM];
+    kind[sl] = (int)q[PR_KIND];
+    nrm[sl] = (int)q[PR_NORMAL];
+    if (kind[sl] == PK_NORMAL) {
       const Real sp = sep + C.slop;
       if (sp > 0.0) {
-        bias1 = bias0 = sp / C.h;
+        bias1[sl] = bias0[sl] = sp / C.h;
       } else {
-        bias1 = rmax(C.bias_rate * sp, -C.max_bias);
-        ms1 = C.mass_scale;
-        is1 = C.impulse_scale;
+        bias1[sl] = rmax(C.bias_rate * sp, -C.max_bias);
+        ms1[sl] = C.mass_scale;
+        is1[sl] = C.impulse_scale;
       }
-    } else if (kind == PK_BILATERAL) {
-      bias1 = rclamp(C.bias_rate * sep, -C.max_bias, C.max_bias);
-      ms1 = C.mass_scale;
-      is1 = C.impulse_scale;
+    } else if (kind[sl] == PK_BILATERAL) {
+      bias1[sl] = rclamp(C.bias_rate * sep, -C.max_bias, C.max_bias);
+      ms1[sl] = C.mass_scale;
+      is1[sl] = C.impulse_scale;
     }
   }
+  // the value row r holds (r wave-uniform)
+  auto at = [&](const Real (&x)[2], int r) { return r < 64 ? lane_read(x[0], r) : lane_read(x[1], r - 64); };
+  auto at_i = [&](const int (&x)[2], int r) {
+    return r < 64 ? __builtin_amdgcn_readlane(x[0], r) : __builtin_amdgcn_readlane(x[1], r - 64);
+  };
   auto v_finite = [&]() { return __ballot(dof_lane && !__builtin_isfinite(vk)) == 0; };
   bool vfin = v_finite();
   auto sweep = [&](bool use_bias) {
-    Real dq = 0.0;  // |dlambda| of this lane's row, if it changed in this sweep
-    bool upd = false;
+    Real dq[2] = {0.0, 0.0};  // |dlambda| of this lane's rows, if they changed in this sweep
+    bool upd[2] = {false, false};
     // row r with its J and M^-1 J^T entries in jb, mb; reloads them with row
     // r + AHEAD. Every lane loads (lanes past nv repeat the last dof, rows past
     // the end repeat the last row), so no branch sits around a load.
     auto row = [&](int r, Real& jb, Real& mb) {
       const int64_t off = (int64_t)(r + AHEAD < nr ? r + AHEAD : nr - 1) * nv + kl;
-      const Real inv_a = lane_read(inva, r);
+      const Real inv_a = at(inva, r);
       const Real p = dof_lane ? jb * vk : 0.0;  // lanes past nv hold 0
       const uint64_t nz = __ballot(dof_lane && jb != 0.0);
       // both loads sit on the path of every row, skipped or not, so that the
@@ -1636,16 +1650,16 @@ __device__ __noinline__ void pgs_sweeps_wave(const PgsConsts C, const Real* Jp, 
           jv = rsum0();
           for (int k = 0; k < nv; k++) jv += lane_read(p, k);
         }
-        const Real lm = lane_read(lam, r);
-        const int kd = __builtin_amdgcn_readlane(kind, r);
+        const Real lm = at(lam, r);
+        const int kd = at_i(kind, r);
         Real nw;
         if (kd == PK_FRICTION) {
-          const Real bound = lane_read(mu, r) * lane_read(lam, __builtin_amdgcn_readlane(nrm, r));
+          const Real bound = at(mu, r) * at(lam, at_i(nrm, r));
           nw = rclamp(lm - inv_a * jv, -bound, bound);
         } else {
-          const Real bias = lane_read(use_bias ? bias1 : bias0, r);
-          const Real ms = lane_read(use_bias ? ms1 : ms0, r);
-          const Real is = lane_read(use_bias ? is1 : is0, r);
+          const Real bias = use_bias ? at(bias1, r) : at(bias0, r);
+          const Real ms = use_bias ? at(ms1, r) : 1.0;
+          const Real is = use_bias ? at(is1, r) : 0.0;
           if (kd == PK_NORMAL) {
             const Real delta = -inv_a * ms * (jv + bias) - is * lm;
             nw = rmax(lm + delta, 0.0);
@@ -1655,10 +1669,17 @@ __device__ __noinline__ void pgs_sweeps_wave(const PgsConsts C, const Real* Jp, 
         }
         const Real dl = nw - lm;
         if (dl != 0.0) {
-          if (lane == r) {
-            lam = nw;
-            dq = fabs(dl);
-            upd = true;
+          const int sl = r < 64 ? 0 : 1;
+          if (lane == r - 64 * sl) {
+            if (sl == 0) {
+              lam[0] = nw;
+              dq[0] = fabs(dl);
+              upd[0] = true;
+            } else {
+              lam[1] = nw;
+              dq[1] = fabs(dl);
+              upd[1] = true;
+            }
           }
           vk += mb * dl;
           vfin = v_finite();
@@ -1684,32 +1705,48 @@ __device__ __noinline__ void pgs_sweeps_wave(const PgsConsts C, const Real* Jp, 
 #pragma unroll
     for (int b = 0; b < AHEAD - 1; b++)
       if (r + b < nr) row(r + b, jbuf[b], mbuf[b]);
-    Real q = upd ? dq / inva : 0.0;
+    Real q = upd[0] ? dq[0] / inva[0] : 0.0;
+    if (upd[1]) q = rmax(q, dq[1] / inva[1]);
     for (int o = 32; o >= 1; o >>= 1) q = rmax(q, __shfl_xor(q, o));
     return q;
   };
+  LaneVel out;
   for (int it = 0; it < iterations; it++)
     if (sweep(true) < C.tol) break;
-  if (dof_lane) vpos[lane] = vk;
+  out.vpos = vk;
   const int relax = iterations / 3 > 2 ? iterations / 3 : 2;
   for (int it = 0; it < relax; it++)
     if (sweep(false) < C.tol) break;
-  if (dof_lane) v[lane] = vk;
-  if (lane < nr) rw[lane * 10 + PR_LAM] = lam;
+  out.v = vk;
+#pragma unroll
+  for (int sl = 0; sl < 2; sl++)
+    if (lane + 64 * sl < nr) rw[(lane + 64 * sl) * 10 + PR_LAM] = lam[sl];
   __syncthreads();
+  return out;
+}
+// Out of line inside the step kernel (whose registers it would otherwise add
+// to); the sweeps kernel inlines the body, so its register budget applies.
+__device__ __noinline__ LaneVel pgs_sweeps_wave(const PgsConsts C, const Real* Jp, const Real* mjp, Real* rwp, int nr_in,
+                                                int nv_in, Real v_in) {
+  return pgs_sweeps_body(C, Jp, mjp, rwp, nr_in, nv_in, v_in);
 }
 #undef PH_LOAD_FENCE
 #endif
 
-// solver.rs:63-251. v: velocity after the smooth update (in, out); vpos:
-// the biased velocity the positions integrate with. Rows, M^-1 J^T and the
-// row records live in HBM; the rows, the M^-1 J^T columns and the constraint
-// force are formed lane-parallel. A Gauss-Seidel sweep is sequential over rows:
-// when one env owns the wavefront it runs on registers (pgs_sweeps_wave),
-// otherwise one lane per env forms each row's force between lane-parallel
-// products and velocity updates.
+// solver.rs:63-251, in three parts: pgs_prepare (rows, M^-1 J^T, warm
+// start), the Gauss-Seidel sweeps, pgs_finish (constraint force, history for
+// the next warm start). v: velocity after the smooth update (in, out); vpos:
+// the biased velocity the positions integrate with. Rows, M^-1 J^T and the row
+// records live in HBM; the rows, the M^-1 J^T columns and the constraint force
+// are formed lane-parallel. A sweep is sequential over rows: when one env owns
+// the wavefront it runs on registers (pgs_sweeps_wave), otherwise one lane per
+// env forms each row's force between lane-parallel products and velocity
+// updates.
+//
+// pgs_prepare returns the row count, 0 when there is nothing to solve (then v,
+// vpos and the history are final), or -1 when the rows overflowed.
 template <class X, class MT>
-PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
+PHD int pgs_prepare(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
   const int nv = M.nv;
   const Real h = M.timestep;
   int* isc = E.si + M.L.i_iscal;
@@ -1805,14 +1842,14 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
     isc[9] = r;
   });
   const int nr = isc[9];
-  if (isc[7] & ST_ROW_OVERFLOW) return;
+  if (isc[7] & ST_ROW_OVERFLOW) return -1;
   if (nr == 0) {
     X::par(nv, [&](int k) {
       vpos[k] = v[k];
       LD(qfrc_c)[k] = 0.0;
     });
     X::par(1, [&](int) { S.npcon[E.e] = 0; });
-    return;
+    return 0;
   }
   // M^-1 J_r^T and the row's inverse effective mass, one row per lane
   X::par(nr, [&](int r) {
@@ -1859,17 +1896,39 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
       if (l != 0.0) v[k] += mj[r * nv + k] * l;
     }
   });
-  // soft-constraint coefficients (solver.rs:44-50)
+  return nr;
+}
+
+// soft-constraint coefficients (solver.rs:44-50) and the sweeps' stopping rule
+template <class MT>
+PHD PgsConsts pgs_consts(const MT& M) {
+  const Real h = M.timestep;
   const Real omega = 2.0 * 3.141592653589793 * M.hertz;
   const Real a1 = 2.0 * M.zeta + h * omega;
   const Real a2 = h * omega * a1;
   const Real a3 = 1.0 / (1.0 + a2);
-  const PgsConsts C{h, M.slop, omega / a1, a2 * a3, a3, 1.0, 1e-5, M.iterations};
+  return PgsConsts{h, M.slop, omega / a1, a2 * a3, a3, 1.0, 1e-5, M.iterations};
+}
+
+// The Gauss-Seidel sweeps over nr > 0 rows (solver.rs:149-245).
+template <class X, class MT>
+PHD void pgs_sweeps(const MT& M, const DState& S, Env& E, int nr, Real* v, Real* vpos) {
+  const int nv = M.nv;
+  Real* J = (S.gjac + E.e * (int64_t)M.nrmax * nv).p;
+  Real* mj = (S.gmj + E.e * (int64_t)M.nrmax * nv).p;
+  Real* rw = (S.grow + E.e * (int64_t)M.nrmax * 10).p;
+  const PgsConsts C = pgs_consts(M);
   bool swept = false;
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (wave_lanes<X>::value == 64) {
-    if (nr <= 64) {
-      pgs_sweeps_wave(C, J, mj, rw, nr, nv, v, vpos);
+    if (nr <= PGS_WAVE_ROWS) {
+      const int lane = (int)threadIdx.x;
+      const LaneVel r = pgs_sweeps_wave(C, J, mj, rw, nr, nv, lane < nv ? v[lane] : 0.0);
+      if (lane < nv) {
+        v[lane] = r.v;
+        vpos[lane] = r.vpos;
+      }
+      __syncthreads();
       swept = true;
     }
   }
@@ -1918,6 +1977,23 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
     for (int it = 0; it < relax; it++)
       if (iterate(false) < C.tol) break;
   }
+}
+
+// The constraint force and the history for the next warm start, after the
+// sweeps over nr > 0 rows.
+template <class X, class MT>
+PHD void pgs_finish(const MT& M, const DState& S, Env& E, int nr) {
+  const int nv = M.nv;
+  const Real h = M.timestep;
+  int* isc = E.si + M.L.i_iscal;
+  auto con = S.con + E.e * (int64_t)M.ncmax * CW;
+  auto cg = S.cgeom + E.e * (int64_t)M.ncmax * 3;
+  Real* J = (S.gjac + E.e * (int64_t)M.nrmax * nv).p;
+  Real* rw = (S.grow + E.e * (int64_t)M.nrmax * 10).p;
+  auto pcon = S.pcon + E.e * (int64_t)M.ncmax * CW;
+  auto pcg = S.pcgeom + E.e * (int64_t)M.ncmax * 3;
+  auto plam = S.plambda + E.e * (int64_t)M.ncmax * 3;
+  const int ncon = isc[0];
   // qfrc_constraint = sum_r J_r lambda_r / h, rows in order
   X::par(nv, [&](int k) {
     Real f = 0.0;
@@ -1933,6 +2009,14 @@ PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
     }
   });
   X::par(1, [&](int) { S.npcon[E.e] = ncon; });
+}
+
+template <class X, class MT>
+PHD void pgs_solve(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
+  const int nr = pgs_prepare<X>(M, S, E, v, vpos);
+  if (nr <= 0) return;
+  pgs_sweeps<X>(M, S, E, nr, v, vpos);
+  pgs_finish<X>(M, S, E, nr);
 }
 
 // ---------------- integrate_pos, forward.rs:459-480 ----------------
@@ -1961,9 +2045,11 @@ enum { STEP_FAILED = -1, STEP_DONE = -2 };
 // The step up to the constraint rows: forward dynamics, collision, the
 // unconstrained acceleration a0 (in LD(a)) and the rows. Returns the row count,
 // STEP_FAILED when the env cannot be stepped (status set) or STEP_DONE when the
-// PGS branch has completed the step.
+// PGS branch has completed the step. With pgs_front_only the PGS branch stops
+// before its sweeps and returns the row count (velocity in LD(vel); with no
+// rows the step's velocities are final and LD(acc) holds vpos).
 template <class X, class MT>
-PHD int step_front(const MT& M, const DState& S, Env& E) {
+PHD int step_front(const MT& M, const DState& S, Env& E, bool pgs_front_only = false) {
   const int nv = M.nv;
   const Real h = M.timestep;
   int* isc = E.si + M.L.i_iscal;
@@ -1994,6 +2080,10 @@ PHD int step_front(const MT& M, const DState& S, Env& E) {
     Real* vel = LD(vel);
     Real* vpos = LD(acc);
     X::par(nv, [&](int k) { vel[k] = LD(qvel)[k] + h * a[k]; });
+    if (pgs_front_only) {
+      const int nr = pgs_prepare<X>(M, S, E, vel, vpos);
+      return nr < 0 ? STEP_FAILED : nr;
+    }
     pgs_solve<X>(M, S, E, vel, vpos);
     if (isc[7] & ST_ROW_OVERFLOW) return STEP_FAILED;
     X::par(nv, [&](int k) { LD(qacc)[k] = (vel[k] - LD(qvel)[k]) / h; });
@@ -2202,6 +2292,109 @@ PHD void run_env(const MT& M, const DState& S, Env& E, int64_t nstep, const Real
     S.cost[e] = cost;
     S.time[e] += (Real)nstep * M.timestep;
   });
+}
+
+// The PGS step split around its Gauss-Seidel sweeps, one physics step (step t
+// of nstep) per launch of each part: run_pgs_front runs the step up to the
+// sweeps and leaves in HBM the velocity (vel_g, the env's nv entries), vpos
+// when there are no rows, and the contact and row counts and status;
+// k_pgs_sweeps runs the sweeps from there; run_pgs_back forms the constraint
+// force and the history, integrates and stores the state, as run_env does for
+// the step. Everything a part reads from another is in HBM or recomputed, so
+// the three give run_env's results bit for bit.
+template <class X, class MT>
+PHD void run_pgs_front(const MT& M, const DState& S, Env& E, int64_t nstep, int64_t t, const Real* ctrl_seq,
+                       int64_t rel, Real* vel_g, Real* vpos_g) {
+  Real* lds = E.s;
+  load_state<X>(M, S, E);
+  int* isc = E.si + M.L.i_iscal;
+  const int64_t e = E.e;
+  if (isc[IS_STATUS] != 0) return;  // a failed env stays put (the reference panics)
+  if (ctrl_seq) {
+    X::par(M.nu, [&](int k) { lds[M.L.ctrl + k] = ctrl_seq[(rel * nstep + t) * M.nu + k]; });
+    X::par(M.nu, [&](int k) { S.ctrl[e * M.nu + k] = lds[M.L.ctrl + k]; });
+  }
+  const int nr = step_front<X>(M, S, E, true);
+  X::par(M.nv, [&](int k) {
+    vel_g[k] = LD(vel)[k];
+    vpos_g[k] = LD(acc)[k];
+  });
+  X::par(1, [&](int) {
+    S.ncon[e] = isc[IS_NCON];
+    S.status[e] = isc[IS_STATUS];
+    S.nrows[e] = nr >= 0 ? nr : isc[IS_NROWS];
+  });
+}
+
+template <class X, class MT>
+PHD void run_pgs_back(const MT& M, const DState& S, Env& E, int64_t nstep, int64_t t, Real* qpos_out, int64_t rel,
+                      const Real* vel_g, const Real* vpos_g) {
+  Real* lds = E.s;
+  int* isc = E.si + M.L.i_iscal;
+  const int64_t e = E.e;
+  const Real h = M.timestep;
+  const int status = S.status[e];
+  if (status == 0) {
+    Real* vel = LD(vel);
+    Real* vpos = LD(acc);
+    X::par(M.nq, [&](int k) { lds[M.L.qpos + k] = S.qpos[e * M.nq + k]; });
+    X::par(M.nv, [&](int k) {
+      lds[M.L.qvel + k] = S.qvel[e * M.nv + k];
+      vel[k] = vel_g[k];
+      vpos[k] = vpos_g[k];
+    });
+    X::par(1, [&](int) { isc[IS_NCON] = S.ncon[e]; });
+    const int nr = S.nrows[e];
+    if (nr > 0) pgs_finish<X>(M, S, E, nr);
+    X::par(M.nv, [&](int k) { LD(qacc)[k] = (vel[k] - LD(qvel)[k]) / h; });
+    integrate_pos<X>(M, E, vpos, h);
+    X::par(M.nv, [&](int k) { LD(qvel)[k] = vel[k]; });
+    if (qpos_out) X::par(M.nq, [&](int k) { qpos_out[(rel * nstep + t) * M.nq + k] = lds[M.L.qpos + k]; });
+    X::par(M.nq, [&](int k) { S.qpos[e * M.nq + k] = lds[M.L.qpos + k]; });
+    X::par(M.nv, [&](int k) {
+      S.qvel[e * M.nv + k] = LD(qvel)[k];
+      S.qacc[e * M.nv + k] = LD(qacc)[k];
+    });
+  }
+  // run_env's per-launch updates (the PGS step leaves the iteration count alone, so its cost is 0)
+  if (t == nstep - 1)
+    X::par(1, [&](int) {
+      S.cost[e] = 0;
+      S.time[e] += (Real)nstep * M.timestep;
+    });
+}
+
+// The sweeps of one env for the split PGS step when it has more rows than the
+// register sweeps hold, on one lane, in the reference's order (solver.rs:149-245).
+PHD void pgs_sweeps_serial(const PgsConsts& C, const Real* J, const Real* mj, Real* rw, int nr, int nv, Real* v,
+                           Real* vpos) {
+  auto sweep = [&](bool use_bias) {
+    Real maxdv = 0.0;
+    for (int r = 0; r < nr; r++) {
+      Real* q = rw + r * 10;
+      const Real inv_a = q[PR_INVA];
+      if (inv_a == 0.0) continue;
+      Real jv = rsum0();
+      for (int k = 0; k < nv; k++) jv += J[r * nv + k] * v[k];
+      const Real lam = q[PR_LAM];
+      const int kind = (int)q[PR_KIND];
+      const Real lam_n = kind == PK_FRICTION ? rw[(int)q[PR_NORMAL] * 10 + PR_LAM] : 0.0;
+      const Real nw = pgs_row_force(C, kind, inv_a, q[PR_SEP], q[PR_MU], lam, lam_n, jv, use_bias);
+      const Real dl = nw - lam;
+      if (dl != 0.0) {
+        maxdv = rmax(maxdv, fabs(dl) / inv_a);
+        for (int k = 0; k < nv; k++) v[k] += mj[r * nv + k] * dl;
+        q[PR_LAM] = nw;
+      }
+    }
+    return maxdv;
+  };
+  for (int it = 0; it < C.iterations; it++)
+    if (sweep(true) < C.tol) break;
+  for (int k = 0; k < nv; k++) vpos[k] = v[k];
+  const int relax = C.iterations / 3 > 2 ? C.iterations / 3 : 2;
+  for (int it = 0; it < relax; it++)
+    if (sweep(false) < C.tol) break;
 }
 
 // ---------------- learned warm start (no reference counterpart) ----------------

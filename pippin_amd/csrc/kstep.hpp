@@ -125,6 +125,78 @@ __global__ void __launch_bounds__(64) k_step_ws(const DModel* __restrict__ Mp, c
   }
 }
 
+// The PGS step split around its sweeps (run_pgs_front, run_pgs_back in
+// kbody.hpp), one env per workgroup of one wavefront, for physics step t of
+// nstep (K = 1): the front and back keep the step kernel's LDS working set; the sweeps
+// kernel needs no LDS and few registers, so many of its wavefronts share a SIMD
+// and hide each other's dependent chains. vel and vpos hold nv doubles per env
+// (indexed by env id).
+template <int K>
+__global__ void __launch_bounds__(64) k_pgs_front(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                                  int64_t begin, int64_t end, int64_t nstep, int64_t t,
+                                                  const Real* ctrl_seq, Real* vel, Real* vpos) {
+  const DModel& M = *Mp;
+  const DState& S = *Sp;
+  extern __shared__ double lds[];
+  const int64_t rel = (int64_t)blockIdx.x;
+  if (begin + rel >= end) return;
+  Env E;
+  E.s = lds;
+  E.si = reinterpret_cast<int*>(E.s + M.L.ndouble);
+  E.e = begin + rel;
+  E.var = M.env_var[E.e];
+  static_assert(K == 1, "one env per workgroup");
+  run_pgs_front<DevXK<1>>(M, S, E, nstep, t, ctrl_seq, rel, vel + E.e * M.nv, vpos + E.e * M.nv);
+}
+
+template <int K>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_pgs_sweeps(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                                   int64_t begin, int64_t end, Real* vel, Real* vpos) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const DModel& M = *Mp;
+  const DState& S = *Sp;
+  const int64_t e = begin + (int64_t)blockIdx.x;
+  if (e >= end || S.status[e] != 0) return;
+  const int nr = S.nrows[e];
+  if (nr <= 0) return;
+  const int nv = M.nv;
+  Real* J = (S.gjac + e * (int64_t)M.nrmax * nv).p;
+  Real* mj = (S.gmj + e * (int64_t)M.nrmax * nv).p;
+  Real* rw = (S.grow + e * (int64_t)M.nrmax * 10).p;
+  Real* v = vel + e * nv;
+  Real* vp = vpos + e * nv;
+  const PgsConsts C = pgs_consts(M);
+  if (nr <= PGS_WAVE_ROWS) {
+    const int lane = (int)threadIdx.x;
+    const LaneVel r = pgs_sweeps_body(C, J, mj, rw, nr, nv, lane < nv ? v[lane] : 0.0);
+    if (lane < nv) {
+      v[lane] = r.v;
+      vp[lane] = r.vpos;
+    }
+  } else if (threadIdx.x == 0) {
+    pgs_sweeps_serial(C, J, mj, rw, nr, nv, v, vp);
+  }
+#endif
+}
+
+template <int K>
+__global__ void __launch_bounds__(64) k_pgs_back(const DModel* __restrict__ Mp, const DState* __restrict__ Sp,
+                                                 int64_t begin, int64_t end, int64_t nstep, int64_t t, Real* qpos_out,
+                                                 const Real* vel, const Real* vpos) {
+  const DModel& M = *Mp;
+  const DState& S = *Sp;
+  extern __shared__ double lds[];
+  const int64_t rel = (int64_t)blockIdx.x;
+  if (begin + rel >= end) return;
+  Env E;
+  E.s = lds;
+  E.si = reinterpret_cast<int*>(E.s + M.L.ndouble);
+  E.e = begin + rel;
+  E.var = M.env_var[E.e];
+  static_assert(K == 1, "one env per workgroup");
+  run_pgs_back<DevXK<1>>(M, S, E, nstep, t, qpos_out, rel, vel + E.e * M.nv, vpos + E.e * M.nv);
+}
+
 // Contact-count bucketing for k_step_ws, one workgroup: envs [begin, end)
 // keyed by the Newton iterations of their last launch (summed over its steps)
 // and their last contact count, grouped by key, costliest key first, into

@@ -258,6 +258,8 @@ struct ph_sim {
   int64_t* reset_scratch = nullptr;  // this device's `scratch` (pippin_hip_reset_noalloc)
   // f32 poses in device memory (pippin_hip_poses_device), allocated on first use
   DBuf<ph_pose> d_pose_geom, d_pose_cam;
+  // the split PGS step's velocities between its kernels (n x nv each), for PGS models
+  DBuf<double> d_pgs_vel, d_pgs_vpos;
   DBuf<int> d_cam_body;
   DBuf<double> d_cam_pos, d_cam_quat;
   bool qpos_shared = false;  // a device qpos pointer was handed out (pippin_hip_state_device)
@@ -467,6 +469,10 @@ extern "C" int pippin_hip_create(const ph_model_desc* desc, int64_t num_envs, co
       s->ws_per_cu[ki][sp] = per_cu;
     }
   HIPCHK(s->d_order.alloc(std::max<int64_t>(num_envs, 1)));
+  if (M.solver_kind == 1) {
+    HIPCHK(s->d_pgs_vel.alloc((size_t)std::max<int64_t>(num_envs * M.nv, 1)));
+    HIPCHK(s->d_pgs_vpos.alloc((size_t)std::max<int64_t>(num_envs * M.nv, 1)));
+  }
   HIPCHK(s->d_counter.alloc(1));
   HIPCHK(s->d_counter.zero());
 
@@ -701,6 +707,20 @@ static hipError_t launch_plain(ph_sim* s, const ph_launch_shape& sh, int64_t beg
   const size_t lds = (size_t)K * s->M.L.bytes;
   const DModel* mp = s->d_model.p;
   const DState* sp = s->d_state.p;
+  if (sh.schedule == PH_SCHED_PGS_SPLIT) {  // three kernels per physics step, one env per workgroup
+    const unsigned grid = (unsigned)(end - begin);
+    double* vel = s->d_pgs_vel.p;
+    double* vpos = s->d_pgs_vpos.p;
+    for (int64_t t = 0; t < nstep; t++) {
+      hipLaunchKernelGGL(k_pgs_front<1>, dim3(grid), dim3(64), lds, s->stream, mp, sp, begin, end, nstep, t, ctrl, vel,
+                         vpos);
+      hipLaunchKernelGGL(k_pgs_sweeps<1>, dim3(grid), dim3(64), 0, s->stream, mp, sp, begin, end, vel, vpos);
+      hipLaunchKernelGGL(k_pgs_back<1>, dim3(grid), dim3(64), lds, s->stream, mp, sp, begin, end, nstep, t, qpos_out,
+                         (const double*)vel, (const double*)vpos);
+      if (hipError_t e = hipGetLastError()) return e;
+    }
+    return hipSuccess;
+  }
   if (sh.schedule == PH_SCHED_GRID) {
     const unsigned grid = (unsigned)((end - begin + K - 1) / K);
     void* args[] = {&mp, &sp, &begin, &end, &nstep, &ctrl, &qpos_out};
@@ -753,7 +773,9 @@ static bool shape_valid(const ph_sim* s, const ph_launch_shape& sh) {
   const int K = sh.envs_per_wave;
   if (K != 1 && K != 2 && K != 4) return false;
   if (sh.specialised != 0 && !(sh.specialised == 1 && K == 4 && SpecAnt::matches(s->M))) return false;
-  if (sh.schedule < PH_SCHED_GRID || sh.schedule > PH_SCHED_WS_BUCKET) return false;
+  if (sh.schedule < PH_SCHED_GRID || sh.schedule > PH_SCHED_PGS_SPLIT) return false;
+  // the split PGS step: PGS models, one env per workgroup, the generic kernels
+  if (sh.schedule == PH_SCHED_PGS_SPLIT && (s->M.solver_kind != 1 || K != 1 || sh.specialised != 0)) return false;
   int dev_lds = 0;
   if (hipDeviceGetAttribute(&dev_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) != hipSuccess) return false;
   return (int64_t)K * s->M.L.bytes <= dev_lds;
@@ -846,7 +868,7 @@ extern "C" int pippin_hip_autotune(ph_sim* s, int32_t rounds, const char* cache_
   s->tune_cand.clear();
   for (int spec : {1, 0})
     for (int K : {4, 2, 1})
-      for (int sched : {PH_SCHED_GRID, PH_SCHED_WS, PH_SCHED_WS_BUCKET}) {
+      for (int sched : {PH_SCHED_GRID, PH_SCHED_WS, PH_SCHED_WS_BUCKET, PH_SCHED_PGS_SPLIT}) {
         ph_launch_shape sh{};
         sh.envs_per_wave = K;
         sh.specialised = spec;

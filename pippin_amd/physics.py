@@ -384,7 +384,7 @@ class HipPhysics:
         """Copy a field from device memory, ordered before the next step."""
         self._check(self.L.pippin_hip_set_device(self.h, FIELD[field], begin, end, C.c_void_p(dev_ptr)))
 
-    SCHED = {"grid": 0, "ws": 1, "ws_bucket": 2}
+    SCHED = {"grid": 0, "ws": 1, "ws_bucket": 2, "pgs_split": 3}
 
     def launch_shape(self):
         """(envs per wavefront, specialised kernel, schedule name) of the plain step."""

@@ -18,6 +18,7 @@ from pippin_amd.workloads import random_pile_xml
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(k, sp, sc) for sp in (1, 0) for k in (4, 2, 1) for sc in ("grid", "ws", "ws_bucket")]
+SPLIT = (1, 0, "pgs_split")  # PGS models only: the step as front, sweeps and back kernels
 
 
 def say(*a):
@@ -59,7 +60,7 @@ def _valid_shapes(arr, n):
     fails the test."""
     g = HipPhysics(arr, n)
     ok = []
-    for shape in SHAPES:
+    for shape in SHAPES + [SPLIT]:
         try:
             g.set_launch_shape(*shape)
         except HipError:
@@ -102,6 +103,7 @@ def test_every_shape_matches_on_a_pile(solver):
     ref = _run(arr, q, n, None, 30, 7, graphs=False)
     shapes = _valid_shapes(arr, n)
     assert len(shapes) >= 3, shapes  # at least one K with every schedule
+    assert (SPLIT in shapes) == (solver == 1), shapes
     for shape in shapes:
         say("pile shape", solver, shape)
         got = _run(arr, q, n, shape, 30, 7)
@@ -154,17 +156,20 @@ def test_graph_replays_match_eager_launches(assets, schedule):
         assert np.array_equal(a[1], b[1]), t
 
 
-def test_rollout_matches_under_every_shape_and_while_autotuning():
+@pytest.mark.parametrize("solver", [0, 1])
+def test_rollout_matches_under_every_shape_and_while_autotuning(solver):
     """Batch::rollout with per-step controls and qpos outputs in device
-    memory, under every shape and during autotuning trials."""
+    memory, under every shape and during autotuning trials (both solvers: the
+    split PGS step takes part for PGS)."""
     import torch
 
     arr = mjcf.load_string(random_pile_xml(29))
+    arr["solver_kind"] = solver
     n, T = 11, 12
     rng = np.random.default_rng(31)
     q0 = None
     outs = []
-    shapes = [None] + [s for s in SHAPES if s[1] == 0] + ["autotune"]
+    shapes = [None] + [s for s in _valid_shapes(arr, n) if s[1] == 0] + ["autotune"]
     for shape in shapes:
         g = HipPhysics(arr, n)
         if q0 is None:
