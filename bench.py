@@ -210,6 +210,64 @@ def warm_start_study(model: str, n: int, sub: int, record: int, train_iters: int
             "kernel_ms_per_step_reference": kr / lr_, "kernel_ms_per_step_learned": km / lm}
 
 
+def warm_start_pgs_study(n: int, record: int, train_iters: int, measure: int, seed: int, pile_seed: int = 29):
+    """The learned warm start for PGS on a six-body pile: the network, trained
+    online on the solved forces of the sim's own contacts (PH_MLP_LOG), warm-
+    starts the contacts that the previous-step matching leaves at zero
+    (PH_MLP_WARM). Recorded over the first `record` steps after the drop, then
+    measured over the next `measure` steps, while contacts still come and go:
+    sweeps per physics step with the reference warm start and with the learned
+    one, each step taken from the same saved state (positions, velocities and
+    contact history), and the step kernel time of each."""
+    import torch
+    from pippin_amd.physics import HipPhysics, load_model
+    from pippin_amd.workloads import random_pile_xml
+
+    arrays = load_model(random_pile_xml(pile_seed), is_xml=True)
+    arrays["solver_kind"] = 1
+    ref, mlp = HipPhysics(arrays, n), HipPhysics(arrays, n)
+    for s in (ref, mlp):
+        s.mlp_init(hidden=64, seed=seed, max_batch=8192)
+    rng = np.random.default_rng(pile_seed)
+    q = ref.get("qpos")
+    for b in range(int(arrays["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    for s in (ref, mlp):
+        s.set("qpos", 0, n, q)
+    ref.mlp_mode(HipPhysics.MLP_LOG)  # the reference warm start; sweep counts reported
+    mlp.mlp_mode(HipPhysics.MLP_LOG)
+    stored = 0
+    for _ in range(record):
+        for s in (ref, mlp):
+            s.step(0, n, 1)
+        stored = mlp.mlp_collect()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = mlp.mlp_train(train_iters, batch=8192, lr=2e-3, seed=seed)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    mlp.mlp_mode(HipPhysics.MLP_WARM)
+    sw_ref, sw_mlp = [], []
+    ref.kernel_timing(reset=True)
+    mlp.kernel_timing(reset=True)
+    for _ in range(measure):
+        mlp.load_state(ref.save_state())
+        ref.step(0, n, 1)
+        mlp.step(0, n, 1)
+        sw_ref.append(ref.solver_iterations())
+        sw_mlp.append(mlp.solver_iterations())
+    kr, lr_ = ref.kernel_timing(reset=True)
+    km, lm = mlp.kernel_timing(reset=True)
+    a, b = np.concatenate(sw_ref), np.concatenate(sw_mlp)
+    stats = lambda x: {"mean": float(x.mean()), "p50": float(np.percentile(x, 50)),
+                       "p99": float(np.percentile(x, 99))}
+    return {"model": f"pile seed {pile_seed} (6 bodies, nv 36), PGS", "envs": n, "recorded_contacts": int(stored),
+            "train_steps": train_iters, "train_s": train_s, "loss_first": float(loss[:10].mean()),
+            "loss_last": float(loss[-10:].mean()), "sweeps_reference": stats(a), "sweeps_learned": stats(b),
+            "sweeps_reduction": float((a.mean() - b.mean()) / a.mean()),
+            "kernel_ms_per_step_reference": kr / lr_, "kernel_ms_per_step_learned": km / lm}
+
+
 def pile_study(n: int, steps: int, seed: int, cpu_envs: int, general: bool = False, solvers=("newton", "pgs")):
     """Tabletop piles (pippin_amd/workloads.py random_pile_xml: 2 to 6 free
     primitives dropped onto a plane), after the reference's tumble scenes:
@@ -507,6 +565,8 @@ def main():
                          "(reference kernel for scaling HBM byte counters), then exit")
     ap.add_argument("--pile-gpu", default="",
                     help="SEED,SOLVER: one pile study (8192 envs, 200 steps) on the GPU only, for profiling")
+    ap.add_argument("--warm-start-pgs", action="store_true",
+                    help="the learned warm start for PGS on the six-body pile: sweeps per step with and without")
     ap.add_argument("--warm-start-wide", action="store_true",
                     help="the learned warm-start study on the six-body pile and the tabletop tumble (Newton)")
     ap.add_argument("--warm-start", action="store_true",
@@ -557,6 +617,10 @@ def main():
                                           ("box_drop.xml", args.envs, 1, 400, 100)):
             log(f"warm-start study: {model}, {n} envs")
             print(json.dumps(warm_start_study(model, n, sub, rec, 1500, meas, args.seed)), flush=True)
+        return
+    if args.warm_start_pgs:
+        log("warm-start study: PGS pile seed 29, 8192 envs")
+        print(json.dumps(warm_start_pgs_study(8192, 100, 1500, 100, args.seed)), flush=True)
         return
     if args.warm_start_wide:
         # the wide Newton workloads: the six-body pile and the tabletop tumble, 8192 envs

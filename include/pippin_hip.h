@@ -353,13 +353,20 @@ int pippin_hip_autotune_pending(const ph_sim* s);
  * and shape repeat (the most recent 8 are kept). */
 int pippin_hip_use_graphs(ph_sim* s, int32_t on);
 
-/* ---- learned Newton warm start ----
- * No counterpart in the reference: its warm start is the previous acceleration,
- * taken when its cost is lower than that of a0 (crates/pippin/src/newton.rs:238-245).
- * A 24 -> H -> H -> H -> 4 GELU MLP (BF16 weights, FP32 accumulation on the
- * matrix cores) predicts the four pyramid-row forces of each of an env's first 16
- * touching contacts; a0 + M^-1 J^T lambda becomes a third candidate, used only
- * if its Newton cost is lower than the reference's choice. Newton solver only. */
+/* ---- learned warm start ----
+ * No counterpart in the reference. A 24 -> H -> H -> H -> 4 GELU MLP (BF16
+ * weights, FP32 accumulation on the matrix cores) reads features of each of an
+ * env's first 16 contacts.
+ * Newton (the reference warm-starts from the previous acceleration when its
+ * cost is lower than that of a0, crates/pippin/src/newton.rs:238-245): the
+ * network predicts the contact's four pyramid-row forces, and a0 + M^-1 J^T
+ * lambda becomes a third candidate, used only if its Newton cost is lower than
+ * the reference's choice.
+ * PGS (the reference warm-starts each contact from the nearest previous
+ * contact of the same geom pair and leaves the others at zero,
+ * crates/pippin/src/solver.rs:253-289): the network predicts the normal and
+ * friction forces, which become the warm start of the contacts the matching
+ * leaves at zero; in MLP mode solver_iterations reports the step's sweeps. */
 #define PH_MLP_LOG 1  /* record features and solved forces of every step */
 #define PH_MLP_WARM 2 /* use the predicted candidate (hidden width <= 64) */
 /* Allocate the network (hidden width H, a multiple of 16 up to 256; weights

@@ -1575,6 +1575,7 @@ __device__ __forceinline__ Real lane_read(Real x, int l) {
 constexpr int AHEAD = 4;  // rows whose J and M^-1 J^T are in flight
 struct LaneVel {
   Real v, vpos;  // this lane's entry of the solved velocity and of the one positions integrate with
+  int sweeps;    // sweeps run, both phases
 };
 constexpr int PGS_WAVE_ROWS = 128;  // rows the register sweeps hold: two per lane
 __device__ __forceinline__ LaneVel pgs_sweeps_body(const PgsConsts C, const Real* Jp, const Real* mjp, Real* rwp,
@@ -1711,12 +1712,17 @@ __device__ __forceinline__ LaneVel pgs_sweeps_body(const PgsConsts C, const Real
     return q;
   };
   LaneVel out;
-  for (int it = 0; it < iterations; it++)
+  out.sweeps = 0;
+  for (int it = 0; it < iterations; it++) {
+    out.sweeps++;
     if (sweep(true) < C.tol) break;
+  }
   out.vpos = vk;
   const int relax = iterations / 3 > 2 ? iterations / 3 : 2;
-  for (int it = 0; it < relax; it++)
+  for (int it = 0; it < relax; it++) {
+    out.sweeps++;
     if (sweep(false) < C.tol) break;
+  }
   out.v = vk;
 #pragma unroll
   for (int sl = 0; sl < 2; sl++)
@@ -1743,10 +1749,26 @@ __device__ __noinline__ LaneVel pgs_sweeps_wave(const PgsConsts C, const Real* J
 // env forms each row's force between lane-parallel products and velocity
 // updates.
 //
-// pgs_prepare returns the row count, 0 when there is nothing to solve (then v,
-// vpos and the history are final), or -1 when the rows overflowed.
+// v += M^-1 J^T lambda over the rows' warm-start forces (solver.rs:155-161).
 template <class X, class MT>
-PHD int pgs_prepare(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
+PHD void pgs_apply_warm(const MT& M, const DState& S, Env& E, int nr, Real* v) {
+  const int nv = M.nv;
+  const Real* mj = (S.gmj + E.e * (int64_t)M.nrmax * nv).p;
+  const Real* rw = (S.grow + E.e * (int64_t)M.nrmax * 10).p;
+  X::par(nv, [&](int k) {
+    for (int r = 0; r < nr; r++) {
+      Real l = rw[r * 10 + PR_LAM];
+      if (l != 0.0) v[k] += mj[r * nv + k] * l;
+    }
+  });
+}
+
+// pgs_prepare returns the row count, 0 when there is nothing to solve (then v,
+// vpos and the history are final), or -1 when the rows overflowed. Without
+// apply_warm it stops with the warm-start forces in the row records, before
+// they move v (pgs_apply_warm).
+template <class X, class MT>
+PHD int pgs_prepare(const MT& M, const DState& S, Env& E, Real* v, Real* vpos, bool apply_warm = true) {
   const int nv = M.nv;
   const Real h = M.timestep;
   int* isc = E.si + M.L.i_iscal;
@@ -1890,12 +1912,7 @@ PHD int pgs_prepare(const MT& M, const DState& S, Env& E, Real* v, Real* vpos) {
       rw[(3 * c + 2) * 10 + PR_LAM] = dot(f, t2);
     }
   });
-  X::par(nv, [&](int k) {
-    for (int r = 0; r < nr; r++) {
-      Real l = rw[r * 10 + PR_LAM];
-      if (l != 0.0) v[k] += mj[r * nv + k] * l;
-    }
-  });
+  if (apply_warm) pgs_apply_warm<X>(M, S, E, nr, v);
   return nr;
 }
 
@@ -1910,15 +1927,15 @@ PHD PgsConsts pgs_consts(const MT& M) {
   return PgsConsts{h, M.slop, omega / a1, a2 * a3, a3, 1.0, 1e-5, M.iterations};
 }
 
-// The Gauss-Seidel sweeps over nr > 0 rows (solver.rs:149-245).
+// The Gauss-Seidel sweeps over nr > 0 rows (solver.rs:149-245); returns the
+// number of sweeps run.
 template <class X, class MT>
-PHD void pgs_sweeps(const MT& M, const DState& S, Env& E, int nr, Real* v, Real* vpos) {
+PHD int pgs_sweeps(const MT& M, const DState& S, Env& E, int nr, Real* v, Real* vpos) {
   const int nv = M.nv;
   Real* J = (S.gjac + E.e * (int64_t)M.nrmax * nv).p;
   Real* mj = (S.gmj + E.e * (int64_t)M.nrmax * nv).p;
   Real* rw = (S.grow + E.e * (int64_t)M.nrmax * 10).p;
   const PgsConsts C = pgs_consts(M);
-  bool swept = false;
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (wave_lanes<X>::value == 64) {
     if (nr <= PGS_WAVE_ROWS) {
@@ -1929,7 +1946,7 @@ PHD void pgs_sweeps(const MT& M, const DState& S, Env& E, int nr, Real* v, Real*
         vpos[lane] = r.vpos;
       }
       __syncthreads();
-      swept = true;
+      return r.sweeps;
     }
   }
 #endif
@@ -1969,14 +1986,18 @@ PHD void pgs_sweeps(const MT& M, const DState& S, Env& E, int nr, Real* v, Real*
     }
     return sc[8];
   };
-  if (!swept) {
-    for (int it = 0; it < M.iterations; it++)
-      if (iterate(true) < C.tol) break;
-    X::par(nv, [&](int k) { vpos[k] = v[k]; });
-    const int relax = M.iterations / 3 > 2 ? M.iterations / 3 : 2;
-    for (int it = 0; it < relax; it++)
-      if (iterate(false) < C.tol) break;
+  int sweeps = 0;
+  for (int it = 0; it < M.iterations; it++) {
+    sweeps++;
+    if (iterate(true) < C.tol) break;
   }
+  X::par(nv, [&](int k) { vpos[k] = v[k]; });
+  const int relax = M.iterations / 3 > 2 ? M.iterations / 3 : 2;
+  for (int it = 0; it < relax; it++) {
+    sweeps++;
+    if (iterate(false) < C.tol) break;
+  }
+  return sweeps;
 }
 
 // The constraint force and the history for the next warm start, after the
@@ -2042,14 +2063,28 @@ PHD void integrate_pos(const MT& M, Env& E, const Real* v, Real h) {
 // ---------------- one step, forward.rs:336-389 (Newton) ----------------
 enum { STEP_FAILED = -1, STEP_DONE = -2 };
 
+// The end of the PGS step (forward.rs:336-389, PGS branch): qacc from the
+// velocity change, positions integrated with vpos (LD(acc)), qvel = LD(vel).
+template <class X, class MT>
+PHD void pgs_step_end(const MT& M, Env& E) {
+  const Real h = M.timestep;
+  Real* vel = LD(vel);
+  X::par(M.nv, [&](int k) { LD(qacc)[k] = (vel[k] - LD(qvel)[k]) / h; });
+  integrate_pos<X>(M, E, LD(acc), h);
+  X::par(M.nv, [&](int k) { LD(qvel)[k] = vel[k]; });
+}
+
 // The step up to the constraint rows: forward dynamics, collision, the
 // unconstrained acceleration a0 (in LD(a)) and the rows. Returns the row count,
 // STEP_FAILED when the env cannot be stepped (status set) or STEP_DONE when the
-// PGS branch has completed the step. With pgs_front_only the PGS branch stops
-// before its sweeps and returns the row count (velocity in LD(vel); with no
-// rows the step's velocities are final and LD(acc) holds vpos).
+// PGS branch has completed the step. With pgs_stop the PGS branch stops
+// early and returns the row count: PGS_BEFORE_SWEEPS with the warm start
+// applied (velocity in LD(vel); with no rows the step's velocities are final
+// and LD(acc) holds vpos), PGS_BEFORE_WARM with the warm-start forces still in
+// the row records.
+enum { PGS_FULL = 0, PGS_BEFORE_SWEEPS = 1, PGS_BEFORE_WARM = 2 };
 template <class X, class MT>
-PHD int step_front(const MT& M, const DState& S, Env& E, bool pgs_front_only = false) {
+PHD int step_front(const MT& M, const DState& S, Env& E, int pgs_stop = PGS_FULL) {
   const int nv = M.nv;
   const Real h = M.timestep;
   int* isc = E.si + M.L.i_iscal;
@@ -2080,15 +2115,13 @@ PHD int step_front(const MT& M, const DState& S, Env& E, bool pgs_front_only = f
     Real* vel = LD(vel);
     Real* vpos = LD(acc);
     X::par(nv, [&](int k) { vel[k] = LD(qvel)[k] + h * a[k]; });
-    if (pgs_front_only) {
-      const int nr = pgs_prepare<X>(M, S, E, vel, vpos);
+    if (pgs_stop != PGS_FULL) {
+      const int nr = pgs_prepare<X>(M, S, E, vel, vpos, pgs_stop == PGS_BEFORE_SWEEPS);
       return nr < 0 ? STEP_FAILED : nr;
     }
     pgs_solve<X>(M, S, E, vel, vpos);
     if (isc[7] & ST_ROW_OVERFLOW) return STEP_FAILED;
-    X::par(nv, [&](int k) { LD(qacc)[k] = (vel[k] - LD(qvel)[k]) / h; });
-    integrate_pos<X>(M, E, vpos, h);
-    X::par(nv, [&](int k) { LD(qvel)[k] = vel[k]; });
+    pgs_step_end<X>(M, E);
     return STEP_DONE;
   }
   if (!factor_solve_mass<X>(M, E, 0.0, a)) {
@@ -2314,7 +2347,7 @@ PHD void run_pgs_front(const MT& M, const DState& S, Env& E, int64_t nstep, int6
     X::par(M.nu, [&](int k) { lds[M.L.ctrl + k] = ctrl_seq[(rel * nstep + t) * M.nu + k]; });
     X::par(M.nu, [&](int k) { S.ctrl[e * M.nu + k] = lds[M.L.ctrl + k]; });
   }
-  const int nr = step_front<X>(M, S, E, true);
+  const int nr = step_front<X>(M, S, E, PGS_BEFORE_SWEEPS);
   X::par(M.nv, [&](int k) {
     vel_g[k] = LD(vel)[k];
     vpos_g[k] = LD(acc)[k];
@@ -2498,6 +2531,96 @@ PHD void mlp_targets(const MT& M, const DState& S, Env& E) {
   });
 }
 
+// The learned warm start for PGS (no reference counterpart; the reference
+// warm-starts each contact from the nearest previous contact of the same geom
+// pair, solver.rs:253-289, and leaves the others at zero). Between the rows and
+// the warm start's velocity update, on every env of a wavefront at once:
+// features of the env's first MLP_CL contacts, the network, and with
+// MLP_WARM, for each of those contacts that the matching left at zero (normal
+// and both friction forces), the predicted forces. Outputs 0..2 are
+// sign(x) log1p(|x|) of 100 lambda_r / inv_a_r, the velocity change (cm/s) each
+// row's force makes along the row, for the normal and the two friction rows;
+// the normal force is clamped at 0 and the friction forces within the rows'
+// bounds (solver.rs:190-196). code as for mlp_stage (here the row count of a
+// PGS env).
+template <class X, class MT>
+PHD void mlp_stage_pgs(const MT& M, const DState& S, Env& E, int code) {
+  const int nv = M.nv;
+  int* isc = E.si + M.L.i_iscal;
+  const int64_t e = E.e;
+  Real* rw = (S.grow + e * (int64_t)M.nrmax * 10).p;
+  int cnt = 0;
+  if (code > 0) {
+    const int ncon = isc[0];
+    cnt = ncon < MLP_CL ? ncon : MLP_CL;
+    const Real* J = (S.gjac + e * (int64_t)M.nrmax * nv).p;
+    const Real* v = LD(vel);
+    auto con = S.con + e * (int64_t)M.ncmax * CW;
+    X::par(cnt, [&](int c) {
+      auto f = S.mlp_feat + (e * MLP_CL + c) * MLP_NF;
+      bool matched = false;
+      for (int j = 0; j < 3; j++) {
+        const int r = 3 * c + j;
+        const Real* q = rw + r * 10;
+        f[j] = (float)mlp_slog(sdot_n(J + r * nv, v, nv));  // row velocity before the warm start
+        f[3 + j] = (float)(0.1 * log(q[PR_INVA] > 0.0 ? q[PR_INVA] : 1e-30));
+        f[6 + j] = (float)mlp_slog(q[PR_LAM]);              // the reference's warm start
+        matched = matched || q[PR_LAM] != 0.0;
+      }
+      auto cc = con + c * CW;
+      f[9] = matched ? 1.0f : 0.0f;
+      f[10] = (float)mlp_slog(1000.0 * cc[9]);  // depth in mm
+      f[11] = (float)rw[3 * c * 10 + PR_MU];
+      f[12] = (float)cc[3];
+      f[13] = (float)cc[4];
+      f[14] = (float)cc[5];
+      f[15] = (float)(ncon / 16.0);
+      f[16] = (float)((code - 3 * ncon) / 16.0);  // limit and equality rows
+      for (int k = 17; k < MLP_NF; k++) f[k] = 0.0f;
+    });
+  }
+  X::par(1, [&](int) {
+    if (code != MLP_ABSENT) S.mlp_cnt[e] = cnt;
+    isc[13] = cnt;
+  });
+  X::mlp_tiles(M, S, e, isc[13]);
+  if (code <= 0 || cnt == 0 || !(M.mlp.mode & MLP_WARM)) return;
+  auto pred = S.mlp_pred + e * MLP_CL * MLP_NO;
+  X::par(cnt, [&](int c) {
+    Real* qn = rw + 3 * c * 10;
+    if (qn[PR_LAM] != 0.0 || qn[10 + PR_LAM] != 0.0 || qn[20 + PR_LAM] != 0.0) return;  // matched
+    auto inv = [&](int j) {  // lambda from 100 lambda / inv_a
+      const Real z = (Real)pred[c * MLP_NO + j];
+      const Real ia = qn[10 * j + PR_INVA];
+      return ia > 0.0 ? (z < 0.0 ? -expm1(-z) : expm1(z)) * 0.01 * ia : 0.0;
+    };
+    Real ln = inv(0);
+    if (!(ln > 0.0)) ln = 0.0;  // also drops a NaN
+    const Real bound = qn[PR_MU] * ln;
+    qn[PR_LAM] = ln;
+    for (int j = 1; j < 3; j++) {
+      const Real t = inv(j);
+      qn[10 * j + PR_LAM] = rfinite(t) ? rclamp(t, -bound, bound) : 0.0;
+    }
+  });
+}
+
+// Training targets after the PGS solve: sign(x) log1p(|x|) of 100 lambda_r /
+// inv_a_r for the normal and friction rows of the recorded contacts (output 3
+// unused; 0 for a row without effective mass).
+template <class X, class MT>
+PHD void mlp_targets_pgs(const MT& M, const DState& S, Env& E) {
+  const int* isc = E.si + M.L.i_iscal;
+  const int cnt = isc[13];
+  const Real* rw = (S.grow + E.e * (int64_t)M.nrmax * 10).p;
+  auto tg = S.mlp_tgt + E.e * MLP_CL * MLP_NO;
+  X::par(cnt * MLP_NO, [&](int k) {
+    const int c = k / MLP_NO, j = k % MLP_NO;
+    const Real* q = rw + (3 * c + (j < 3 ? j : 0)) * 10;
+    tg[k] = j < 3 && q[PR_INVA] > 0.0 ? (float)mlp_slog(100.0 * q[PR_LAM] / q[PR_INVA]) : 0.0f;
+  });
+}
+
 // run_env with the learned warm start. Every lane group of the wavefront
 // reaches mlp_stage once per step, whether its env steps, has failed or does
 // not exist, because the network runs on the whole wavefront.
@@ -2513,7 +2636,27 @@ PHD void run_env_mlp(const MT& M, const DState& S, Env& E, int64_t nstep, const 
     int code = live ? (int)STEP_FAILED : (int)MLP_ABSENT;
     if (go) {
       if (ctrl_seq) X::par(M.nu, [&](int k) { lds[M.L.ctrl + k] = ctrl_seq[(rel * nstep + t) * M.nu + k]; });
-      code = step_front<X>(M, S, E);
+      code = step_front<X>(M, S, E, PGS_BEFORE_WARM);
+    }
+    if (M.solver_kind == 1) {
+      // PGS: the network sets the warm start of unmatched contacts; the step's
+      // sweep count goes to the iteration counter
+      mlp_stage_pgs<X>(M, S, E, code);
+      if (code >= 0) {
+        Real* vel = LD(vel);
+        Real* vpos = LD(acc);
+        int sweeps = 0;
+        if (code > 0) {
+          pgs_apply_warm<X>(M, S, E, code, vel);
+          sweeps = pgs_sweeps<X>(M, S, E, code, vel, vpos);
+          pgs_finish<X>(M, S, E, code);
+          if (M.mlp.mode & MLP_LOG) mlp_targets_pgs<X>(M, S, E);
+        }
+        pgs_step_end<X>(M, E);
+        X::par(1, [&](int) { isc[IS_ITERS] = sweeps; });
+        if (qpos_out) X::par(M.nq, [&](int k) { qpos_out[(rel * nstep + t) * M.nq + k] = lds[M.L.qpos + k]; });
+      }
+      continue;
     }
     const Real* w2 = mlp_stage<X>(M, S, E, code);
     bool ok = go && code == STEP_DONE;

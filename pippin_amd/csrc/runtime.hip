@@ -1597,7 +1597,6 @@ static int mlp_check(ph_sim* s) {
 
 extern "C" int pippin_hip_mlp_init(ph_sim* s, int32_t hidden, uint64_t seed, int64_t capacity, int64_t max_batch) {
   if (!s) return fail(PH_ERR_ARG, "null sim");
-  if (s->M.solver_kind != 0) return fail(PH_ERR_ARG, "mlp_init: the learned warm start is for the Newton solver");
   if (hidden < 16 || hidden > MLP_HMAX || hidden % 16)
     return fail(PH_ERR_ARG, "mlp_init: hidden width must be a multiple of 16 in [16, 256]");
   if (max_batch < 16 || max_batch % 16) return fail(PH_ERR_ARG, "mlp_init: max_batch must be a positive multiple of 16");

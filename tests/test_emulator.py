@@ -492,3 +492,91 @@ def test_urdf_arm_exact(tmp_path, solver):
         assert np.array_equal(o.get("qpos"), e.get("qpos")), t
         seen += int(o.ncon().sum())
     assert seen > 0
+
+
+def _pgs_pile_pair(n, seed=29):
+    from pippin_amd.workloads import random_pile_xml
+
+    arr = mjcf.load_string(random_pile_xml(seed))
+    arr["solver_kind"] = 1
+    o, e = OracleSim(arr, n), EmuSim(arr, n)
+    rng = np.random.default_rng(seed)
+    q = o.get("qpos")
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+    return arr, o, e
+
+
+def _slog(x):
+    return np.sign(x) * np.log1p(np.abs(x))
+
+
+def test_pgs_learned_warm_start_logging_is_the_reference_path():
+    """The PGS hook's kernel body (run_env_mlp's PGS branch, mlp_stage_pgs,
+    mlp_targets_pgs) on the host with logging on: trajectories equal the
+    oracle's bit for bit; the records hold the env's first 16 contacts, their
+    targets are sign(x) log1p(|x|) of the solved normal and friction forces
+    scaled by 100 / inv_a (normal >= 0, friction within mu times normal),
+    and the predictions equal
+    the FP64 restatement of the network on the recorded features."""
+    from oracle import mlp as omlp
+
+    n, h = 4, 32
+    arr, o, e = _pgs_pile_pair(n)
+    P = omlp.random_params(h, np.random.default_rng(5), scale=1.5)
+    e.mlp_set(h, 1, P)
+    seen = 0
+    for t in range(120):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        for k in range(n):
+            f, pr, tg = e.mlp_records(k)
+            assert f.shape[0] == min(int(o.ncon()[k]), 16), (t, k)
+            if f.shape[0]:
+                seen += f.shape[0]
+                assert np.isfinite(f).all() and np.isfinite(tg).all()
+                assert (tg[:, 0] >= 0).all() and (tg[:, 3] == 0).all()
+                # targets are sign(x) log1p(|x|) of 100 lambda_r / inv_a_r; the
+                # features carry 0.1 log(inv_a_r) in float32
+                mu = f[:, 11].astype(np.float64)
+                ia = np.exp(10.0 * f[:, 3:6].astype(np.float64))
+                lam = np.sign(tg[:, :3]) * np.expm1(np.abs(tg[:, :3].astype(np.float64))) * 0.01 * ia
+                for j in (1, 2):
+                    assert (np.abs(lam[:, j]) <= mu * lam[:, 0] * (1 + 1e-3) + 1e-9).all(), (t, k)
+                # the FP64 restatement rounds at the same BF16 points; a value on a
+                # rounding boundary may still land one BF16 step apart (2^-8)
+                ref, _ = omlp.forward(P, h, f, bf16=True)
+                assert np.abs(pr - ref).max() <= 1e-3 * (1 + np.abs(ref).max()), (t, k)
+    assert seen > 0
+    assert (e.solver_iterations() > 0).any()  # sweeps of the last step
+
+
+def test_pgs_learned_warm_start_zero_network_and_random_network():
+    """With the hook on: an all-zero network predicts zero forces, so the
+    unmatched contacts start at zero as in the reference and trajectories equal
+    the oracle's bit for bit; with a random network each step from the oracle's
+    positions and velocities stays finite and close to the oracle's step (the
+    warm start moves where the sweeps stop, not what they solve)."""
+    from oracle import mlp as omlp
+
+    n, h = 4, 32
+    arr, o, e = _pgs_pile_pair(n)
+    e.mlp_set(h, 3, np.zeros(omlp.nparam(h), dtype=np.float32))
+    for t in range(100):
+        for s in (o, e):
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+    e.mlp_set(h, 3, omlp.random_params(h, np.random.default_rng(6), scale=0.5))
+    moved = 0.0
+    for t in range(30):
+        e.set("qpos", 0, n, o.get("qpos"))
+        e.set("qvel", 0, n, o.get("qvel"))
+        for s in (o, e):
+            s.step(0, n, 1)
+        d = np.abs(o.get("qvel") - e.get("qvel")).max()
+        moved = max(moved, d)
+        assert np.isfinite(e.get("qvel")).all() and d < 0.05, (t, d)

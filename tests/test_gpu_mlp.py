@@ -175,3 +175,40 @@ def test_online_training_lowers_loss_and_iterations(assets):
     # training and stepping are deterministic, so the reduction is a measured
     # constant of this setup (2.081 -> 2.031 iterations, 2.4 %); it must not vanish
     assert (a - b) / a >= 0.01, (a, b)
+
+
+def test_pgs_hook_with_a_zero_network_is_the_reference_path():
+    """The learned warm start for PGS (run_env_mlp's PGS branch on the device):
+    with an all-zero network the unmatched contacts start at zero as in the
+    reference, so a six-body PGS pile equals the oracle bit for bit over 150
+    steps with the hook and logging on; the records' normal-force targets are
+    >= 0 and the step reports its sweeps."""
+    from oracle.pyoracle import OracleSim
+    from pippin_amd.workloads import random_pile_xml
+
+    arr = mjcf.load_string(random_pile_xml(29))
+    arr["solver_kind"] = 1
+    n = 64
+    o, g = OracleSim(arr, n), HipPhysics(arr, n)
+    g.mlp_init(hidden=32, seed=0, max_batch=1024)
+    g.mlp_set_params(np.zeros(g.mlp_params().size, dtype=np.float32))
+    g.mlp_mode(HipPhysics.MLP_LOG | HipPhysics.MLP_WARM)
+    rng = np.random.default_rng(29)
+    q = o.get("qpos")
+    for b in range(int(arr["nq"]) // 7):
+        q[:, 7 * b:7 * b + 2] += rng.normal(size=(n, 2)) * 0.005
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    seen = 0
+    for t in range(150):
+        for s in (o, g):
+            s.step(0, n, 1)
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+        if t % 25 == 24:
+            for k in range(0, n, 8):
+                f, pr, tg = g.mlp_records(k)
+                assert f.shape[0] == min(int(o.ncon()[k]), 16), (t, k)
+                seen += f.shape[0]
+                assert (tg[:, 0] >= 0).all() and np.isfinite(tg).all()
+    say("pgs hook: records seen", seen, "sweeps mean", g.solver_iterations().mean())
+    assert seen > 0 and (g.solver_iterations() > 0).any()
