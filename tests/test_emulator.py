@@ -72,6 +72,36 @@ def test_ant_trajectories_exact(ant):
     assert contacts > 0 and not e.status().any()
 
 
+def test_ant_pressed_into_the_floor_exact(ant):
+    """Ant states pushed into the floor with bent legs, so that steps hold 17
+    to 32 constraint rows and over 32 rows (more than the 16-row lane solver
+    and LDS row block hold: the dense solver, rows in HBM): trajectories,
+    contact counts and iteration counts equal the oracle's."""
+    n, T = 64, 20
+    o, e = OracleSim(ant, n), EmuSim(ant, n)
+    rng = np.random.default_rng(1)
+    q = o.get("qpos")
+    q[:, 2] = rng.uniform(0.05, 0.35, n)
+    q[:, 7:] += rng.uniform(-0.8, 0.8, (n, 8))
+    for s in (o, e):
+        s.set("qpos", 0, n, q)
+    mid = wide = 0
+    for t in range(T):
+        c = rng.uniform(-1, 1, (n, 8))
+        for s in (o, e):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        it, nr, _ = o.step_stats()
+        mid += int(((nr > 16) & (nr <= 32)).sum())
+        wide += int((nr > 32).sum())
+        assert np.array_equal(o.ncon(), e.ncon()), t
+        assert np.array_equal(it, e.solver_iterations()), t
+        assert np.array_equal(o.get("qpos"), e.get("qpos")), t
+        assert np.array_equal(o.get("qvel"), e.get("qvel")), t
+    assert mid > 20 and wide > 0, (mid, wide)
+    assert not e.status().any()
+
+
 @pytest.mark.parametrize("name", ["cartpole.xml", "pendulum.xml", "tumble.xml", "arm3d.xml"])
 def test_contact_free_models_exact(assets, name):
     """Contact-free models: hinge/slide chains, a free body with a hinged
