@@ -139,6 +139,40 @@ def test_primitive_pairs_match(name):
             assert np.array_equal(co[f], cg[f]), f
 
 
+@pytest.mark.parametrize("shape", [(4, 1, "ws_bucket"), (1, 0, "grid")])
+def test_ant_pressed_into_the_floor_matches(assets, shape):
+    """Ant states pushed into the floor with bent legs: steps with 17 to 32
+    and over 32 constraint rows, more than the 16-row lane solver and LDS row
+    block hold, so they take the dense solver with their rows in HBM. The
+    Ant-specialised kernel at 4 envs per wavefront and the generic one at 1
+    match the oracle bit for bit (contacts, iterations, qpos, qvel)."""
+    n, T = 256, 20
+    arr, o, g = _ant(assets, n)
+    g.set_launch_shape(*shape)
+    rng = np.random.default_rng(1)
+    q = o.get("qpos")
+    q[:, 2] = rng.uniform(0.05, 0.35, n)
+    q[:, 7:] += rng.uniform(-0.8, 0.8, (n, 8))
+    for s in (o, g):
+        s.set("qpos", 0, n, q)
+    mid = wide = 0
+    say("ant pressed into the floor", shape)
+    for t in range(T):
+        c = rng.uniform(-1, 1, (n, 8))
+        for s in (o, g):
+            s.set("ctrl", 0, n, c)
+            s.step(0, n, 1)
+        it, nr, _ = o.step_stats()
+        mid += int(((nr > 16) & (nr <= 32)).sum())
+        wide += int((nr > 32).sum())
+        assert np.array_equal(o.ncon(), g.ncon()), t
+        assert np.array_equal(it, g.solver_iterations()), t
+        assert np.array_equal(o.get("qpos"), g.get("qpos")), t
+        assert np.array_equal(o.get("qvel"), g.get("qvel")), t
+    say("  steps with 17..32 rows", mid, "over 32", wide)
+    assert mid > 50 and wide > 0, (mid, wide)
+
+
 def test_specialised_kernel_equals_generic(assets, monkeypatch):
     """Ant runs a kernel built with compile-time sizes and layout; it must give
     the generic kernel's results bit for bit (and both the oracle's)."""
