@@ -263,7 +263,8 @@ int pippin_hip_appearance(const ph_sim* s, int64_t begin, int64_t end, ph_appear
 int pippin_hip_ncon(const ph_sim* s, int64_t begin, int64_t end, int32_t* out, int64_t len);
 /* Contacts of env `env`; writes at most `cap`, returns the count in *n. */
 int pippin_hip_contacts(const ph_sim* s, int64_t env, ph_contact* out, int32_t cap, int32_t* n);
-/* Newton iterations of the last step per env (Data::solver_iterations, data.rs:53-55). */
+/* Newton iterations of the last step per env (Data::solver_iterations, data.rs:53-55);
+ * with the learned warm start on a PGS model, the sweeps of the last step. */
 int pippin_hip_solver_iterations(const ph_sim* s, int64_t begin, int64_t end, int32_t* out, int64_t len);
 /* Run forward dynamics (kinematics, CRBA, RNE; no stepping) and copy M(q)
  * (envs x nv x nv) and c(q, qdot) = qfrc_bias (envs x nv). */
@@ -396,7 +397,9 @@ int pippin_hip_mlp_forward(ph_sim* s, const float* x, int64_t n, float* y);
  * parameters (num_params floats); no update. */
 int pippin_hip_mlp_loss_grad(ph_sim* s, const float* x, const float* t, int64_t n, float* grad, float* loss);
 /* Records of env `env` from the last step: features (16 x 24), predictions and
- * targets (16 x 4 each, log1p(lambda / D)) and the number of contacts recorded. */
+ * targets (16 x 4 each; Newton: log1p(lambda / D) of the four pyramid rows; PGS:
+ * sign(x) log1p(|x|) of 100 lambda / inv_a for the normal and two friction rows,
+ * output 3 zero) and the number of contacts recorded. */
 int pippin_hip_mlp_records(ph_sim* s, int64_t env, float* feat, float* pred, float* tgt, int32_t* count);
 
 #ifdef __cplusplus
