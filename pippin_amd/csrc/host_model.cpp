@@ -490,11 +490,13 @@ void init_dmodel_scalars(const HostModel& hm, int solver, int max_contacts, phk:
   M.isl_fstride = M.nblock > 1 ? d.nv * d.nv + M.nrmax * (d.nv + 2) + 4 * d.nv : 0;
   M.isl_istride = M.nblock > 1 ? 2 * M.nrmax + d.nv + 2 * M.nblock + 1 : 0;  // ints, in LDS (Layout::i_isl)
   // Rows beyond nrl live in HBM. Keep up to 32 in LDS, fewer if that keeps
-  // the per-env working set within 10 KiB (four 4-env workgroups per CU).
+  // the per-env working set within 10 KiB (four 4-env workgroups per CU); one
+  // row at a time, so that a step just past 16 rows still finds them in LDS
+  // when the budget has room (Ant: 18).
   M.nrl = std::min(M.nrmax, 32);
   M.L = make_layout(hm, M.nrl, M.npair, M.ncmax, M.isl_istride);
   while (M.L.bytes > 10 * 1024 && M.nrl > 16) {
-    M.nrl -= 4;
+    M.nrl -= 1;
     M.L = make_layout(hm, M.nrl, M.npair, M.ncmax, M.isl_istride);
   }
 
