@@ -41,8 +41,8 @@ def arm_gripper_xml(seed: int) -> str:
     U[0.5, 3] kg, position actuators kp = 100) with a two-finger gripper whose
     slide fingers are coupled by a fixed tendon and a joint equality, over a
     cube (half-extent U[0.02, 0.03] m, friction U[0.6, 1.0]) resting on a
-    table; implicitfast. After the survey's arm configuration, which has no
-    model file in the reference."""
+    table; implicitfast. The reference ships no model file for this arm
+    workload, so it is generated here."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
