@@ -2,7 +2,10 @@
 
 Every source is compiled with -ffp-contract=off so the device arithmetic keeps
 the reference's unfused multiply/add order, and with packed-FP32 instructions
-disabled.
+disabled. Device code is scheduled with LLVM's iterative ILP strategy: the step
+kernels run one wavefront per SIMD, so their latency is hidden only by
+independent instructions within the wavefront (the Ant step kernel measured
+about 7 % faster than with the default scheduler; scheduling moves no rounding).
 """
 from __future__ import annotations
 
@@ -21,6 +24,7 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"--offload-arch={ARCH}",
+    "-mllvm", "-amdgpu-sched-strategy=iterative-ilp",
 ]
 
 
