@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # the steady-contact window), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # scaled by a known-size copy (profiles/k_step_ws_ant16k.md); other workloads
 # report null.
-TRAFFIC_BYTES_PER_LAUNCH = 51.1e6
+TRAFFIC_BYTES_PER_LAUNCH = 51.2e6
 TRAFFIC_SOURCE = "profiles/k_step_ws_ant16k.md"
 PREROLL_ACTIONS = 250  # untimed: the timed window starts at action 250 of 1000 (steady contact)
 ERR_BOUND = 1e-9
